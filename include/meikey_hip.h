@@ -1,0 +1,170 @@
+/*
+ * meikey_hip — C ABI of the MI355X (gfx950) Ed25519 vanity-key search backend.
+ *
+ * Drop-in boundary for meikey's GPU backend. Each entry point names the reference
+ * interface it replaces (paths relative to the meikey sources):
+ *   - the device lifecycle and status rows of `Gpu` (src/service/gpu.rs:1127-1238,
+ *     1251-1326) and `Kernel::load` (gpu.rs:678-745);
+ *   - the private `trait Engine { keys_per_iter, prepare, launch }` that `search_with`
+ *     drives (gpu.rs:807-887, 917-968);
+ *   - the self-check / derivation kernels `meikey_derive` and `meikey_pooled`
+ *     (src/service/gpu/kernel.cu:2037-2069, gpu.rs:747-801).
+ * Batched multi-search and serving have no counterpart in the reference; they reuse the
+ * same kernels and are declared at the end.
+ *
+ * Conventions: plain C types only; the caller owns every host array and passes
+ * pointer + length; the library owns all device memory. Every function returns an
+ * mk_status (0 = success); mk_last_error() gives the text of the last failure on the
+ * calling thread. Calls on one mk_gpu must come from one host thread at a time
+ * (the reference drives each GPU from one worker thread, src/service/search.rs:148-153).
+ */
+#ifndef MEIKEY_HIP_H
+#define MEIKEY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, mapped from the reference's GpuError (gpu.rs:445-499). */
+typedef enum {
+    MK_OK = 0,
+    MK_ERR_DRIVER_NOT_FOUND = 1, /* GpuError::DriverNotFound: no HIP runtime / driver */
+    MK_ERR_NO_DEVICE = 2,        /* GpuError::NoDevice */
+    MK_ERR_OUT_OF_RANGE = 3,     /* GpuError::OutOfRange: width / batch / blocks outside their ranges */
+    MK_ERR_SELF_CHECK = 4,       /* GpuError::SelfCheck: device results differ from the known answers */
+    MK_ERR_DRIVER = 5,           /* GpuError::Driver: a HIP call failed */
+    MK_ERR_INVALID = 6,          /* bad argument (null pointer, bad pattern table, ...) */
+    MK_ERR_OVERFLOW = 7          /* more hits than the caller's buffer or the device lists hold */
+} mk_status;
+
+/* Seed sources (src/model/search.rs:33-42). */
+#define MK_SEED_CHACHA20 0 /* default ("secure"): ChaCha20 block index/2, half index&1 */
+#define MK_SEED_COUNTER 1  /* --low-security: last 8 bytes of the key + index */
+
+/* Match modes (kernel.cu:1774-1776, src/model/pattern.rs:57-66). */
+#define MK_MODE_CONTAINS 0
+#define MK_MODE_SUFFIX 1
+#define MK_MODE_REGEX 2
+
+/* Field layouts (--gpu-limbs). The reference's codes are "10", "10u", "8", "10k", "10ku"
+ * and "5f" (gpu.rs:70-106); this backend implements 10 x 25.5-bit limbs with the table
+ * stored as limbs (code "10u"), and accepts "10" as the same layout. */
+#define MK_LIMBS_R25 10
+
+/* Kernel settings: --gpu-width / --gpu-batch / --gpu-limbs (src/controller/cli.rs:85-111,
+ * gpu.rs:185-240 GpuOptions). */
+typedef struct {
+    uint32_t width;  /* comb table width W, 2..24 (reference 2..20); table = rows(W) x (2^(W-1)+1) x 128 B */
+    uint32_t batch;  /* keys per thread per tile, 1..256 (one inversion per `batch` keys per lane) */
+    uint32_t limbs;  /* MK_LIMBS_R25 */
+    uint32_t blocks; /* resident waves per SIMD the grid is sized for, 1..8 (reference: blocks per SM) */
+} mk_options;
+
+/* Device status (gpu.rs:1251-1326 status rows). */
+typedef struct {
+    char name[64];          /* gcnArchName */
+    uint32_t compute_units;
+    uint32_t grid_waves;    /* workgroups (one wave64 each) per launch */
+    uint64_t hbm_bytes;
+    uint64_t table_bytes;
+    uint64_t scratch_bytes; /* per-lane numerator/denominator buffer */
+    mk_options options;
+    uint32_t rows, cols;    /* comb geometry of the table */
+} mk_info;
+
+/* One exact hit: the key at `index` matches with its real x sign. */
+typedef struct {
+    uint64_t index;
+    uint32_t search; /* search number within a batch (0 for single searches) */
+    uint32_t pad;
+    uint8_t seed[32];
+    uint8_t pk[32];
+} mk_hit_rec;
+
+typedef struct mk_gpu mk_gpu;
+typedef struct mk_launch mk_launch;
+
+/* Text of the last failure on this thread (copied into buf, NUL-terminated). */
+void mk_last_error(char* buf, size_t len);
+
+/* Number of HIP devices (device_count, gpu.rs:1127-1140). */
+mk_status mk_device_count(int* n);
+
+/* Default options (W = 16, batch 32, R25 limbs, 2 waves per SIMD). */
+void mk_default_options(mk_options* out);
+
+/* Open device `ordinal`: build the comb table of `opt->width` on the device and run the
+ * known-answer self-check (Gpu::new + Kernel::load + self_check, gpu.rs:1162, 678-745). */
+mk_status mk_open(int ordinal, const mk_options* opt, mk_gpu** out);
+mk_status mk_info_get(const mk_gpu* g, mk_info* out);
+mk_status mk_close(mk_gpu* g);
+
+/* Engine::keys_per_iter (gpu.rs:816, 832-834): keys one launch covers per `iters` step. */
+uint64_t mk_keys_per_iter(const mk_gpu* g);
+
+/* Engine::prepare (gpu.rs:818-825, 836-855). `key` is the search's 32-byte ChaCha20 key
+ * (or counter base). For MK_MODE_CONTAINS / MK_MODE_SUFFIX, `needle` is the text
+ * (lower-cased by the caller when fold = 1, as Needle::new does, gpu.rs:627-659) and
+ * needle_len its length (1..43; 0 = match nothing, for benchmarks). For MK_MODE_REGEX,
+ * `needle` is the DFA table (states x 65 x u16 LE, Dfa::gpu_table, dfa.rs:187-200)
+ * and needle_len the number of states. */
+mk_status mk_prepare(mk_gpu* g, const uint8_t key[32], int seed_source, const uint8_t* needle, uint32_t needle_len,
+                     uint32_t fold, uint32_t mode, uint32_t stop_on_hit, mk_launch** out);
+
+/* Engine::launch (gpu.rs:826, 857-887): search indices [base, base + keys_per_iter * iters)
+ * and return every exact hit, ascending by index (the reference returns one maybe-hit;
+ * this returns the confirmed set, x sign included). With stop_on_hit, the launch may
+ * stop early after the first candidate. *n_hits is the total number of exact hits;
+ * MK_ERR_OVERFLOW when it exceeds `cap` or the device lists. */
+mk_status mk_launch_run(mk_launch* l, uint64_t base, uint32_t iters, mk_hit_rec* hits, uint32_t cap,
+                        uint32_t* n_hits);
+mk_status mk_free_launch(mk_launch* l);
+
+/* meikey_derive (kernel.cu:2062-2069): compressed public keys (with the x sign) of indices
+ * first .. first+n-1; pks = n x 32 bytes, seeds (optional, may be NULL) = n x 32 bytes. */
+mk_status mk_derive(mk_gpu* g, const uint8_t key[32], int seed_source, uint64_t first, uint64_t n, uint8_t* pks,
+                    uint8_t* seeds);
+
+/* Public keys of explicit seeds (n x 32 bytes in, n x 32 bytes out). */
+mk_status mk_derive_seeds(mk_gpu* g, const uint8_t* seeds, uint32_t n, uint8_t* pks);
+
+/* Raw comb-table entries (row, k) as stored on the device: 32 words each (three 10-limb
+ * field elements, 2 pad words). For tests of the table build. */
+mk_status mk_table_read(mk_gpu* g, uint32_t row, uint32_t k, uint32_t count, uint32_t* words);
+
+/* ------------------------------------------------------------------------------------
+ * Batched multi-search: many searches, one persistent launch (no reference counterpart).
+ * Each search has its own key, seed source, pattern and index range; all share the
+ * comb table and the kernel. Hits carry the search number (position in `specs`). */
+typedef struct {
+    uint8_t key[32];
+    int32_t seed_source;
+    uint32_t mode;
+    uint32_t fold;
+    uint32_t needle_len;
+    const uint8_t* needle; /* text or DFA table, as in mk_prepare */
+    uint32_t stop_on_hit;
+    uint32_t pad;
+    uint64_t first; /* first index */
+    uint64_t count; /* number of indices; rounded up to whole tiles of 64 * batch keys */
+} mk_search_spec;
+
+/* Run all `n` searches to completion in one launch. tries (optional, n entries) receives
+ * the number of indices each search covered. */
+mk_status mk_batch_run(mk_gpu* g, const mk_search_spec* specs, uint32_t n, mk_hit_rec* hits, uint32_t cap,
+                       uint32_t* n_hits, uint64_t* tries);
+
+/* Keys one tile covers (64 lanes x batch). */
+uint64_t mk_tile_keys(const mk_gpu* g);
+
+/* Device time of the last launch's search kernel and confirm kernel, in ms, from HIP
+ * events recorded on the library's stream around each kernel. */
+mk_status mk_last_timing(const mk_gpu* g, float* search_ms, float* confirm_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

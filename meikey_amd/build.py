@@ -1,0 +1,53 @@
+"""Build the gfx950 search library and its CPU test build, in-tree.
+
+libmeikey_hip.so      hipcc --offload-arch=gfx950: kernels + C ABI (include/meikey_hip.h)
+libmeikey_hostcheck.so  g++: the same arithmetic headers compiled for the host, test hooks only
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmeikey_hip.so")
+HOSTCHECK = os.path.join(HERE, "libmeikey_hostcheck.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# gfx950 only; packed-FP32 instructions stay off (the kernels are integer code anyway).
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HIP_SOURCES = [os.path.join(CSRC, "mk_kernels.hip"), os.path.join(CSRC, "mk_host.cpp")]
+
+
+def _run(cmd: list[str]) -> None:
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout)
+        raise RuntimeError(f"build failed ({res.returncode}): {' '.join(cmd)}")
+
+
+def build_hip(out: str = LIB) -> str:
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, *HIP_SOURCES])
+    os.replace(tmp, out)
+    return out
+
+
+def build_hostcheck(out: str = HOSTCHECK) -> str:
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-function", "-o", tmp,
+          os.path.join(CSRC, "mk_hostcheck.cpp")])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    build_hostcheck()
+    build_hip()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB, HOSTCHECK)

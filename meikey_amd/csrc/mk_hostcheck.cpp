@@ -1,0 +1,138 @@
+// CPU build of the kernel arithmetic, for tests without a GPU.
+//
+// The headers mk_field.h, mk_hash.h, mk_point.h, mk_match.h and mk_tile.h are the
+// exact code the gfx950 kernels run; compiled here with g++ they expose test hooks so
+// the CPU test suite can compare every stage (seeds, scalar, field operations at their
+// limb bounds, digits, table entries, the search lane's batched inversion, matching)
+// with the oracle. This library is test infrastructure: nothing in the product loads it.
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "mk_field.h"
+#include "mk_hash.h"
+#include "mk_match.h"
+#include "mk_point.h"
+#include "mk_tile.h"
+
+#define MK_MATCHER_FAIL(st, ...) (fprintf(stderr, __VA_ARGS__), fputc('\n', stderr), (st))
+#include "mk_matcher_host.h"
+
+extern "C" {
+
+void hc_seed(const u32 key[8], u32 source, u64 index, u32 seed[8]) {
+    u32 block[16];
+    mk_seed_at(key, source, index, block, true, seed);
+}
+
+void hc_secret_scalar(const u32 seed[8], u32 s[8]) { mk_secret_scalar(seed, s); }
+
+void hc_fe_mul_limbs(const u32 f[10], const u32 g[10], u32 out[10]) {
+    fe a, b;
+    memcpy(a.v, f, 40);
+    memcpy(b.v, g, 40);
+    const fe r = fe_mul(a, b);
+    memcpy(out, r.v, 40);
+}
+
+void hc_fe_sq_limbs(const u32 f[10], u32 out[10]) {
+    fe a;
+    memcpy(a.v, f, 40);
+    const fe r = fe_sq(a);
+    memcpy(out, r.v, 40);
+}
+
+void hc_fe_to_words(const u32 f[10], u32 out[8]) {
+    fe a;
+    memcpy(a.v, f, 40);
+    fe_to_words(a, out);
+}
+
+void hc_fe_from_words(const u32 w[8], u32 out[10]) {
+    const fe a = fe_from_words(w);
+    memcpy(out, a.v, 40);
+}
+
+// op: 0 add, 1 sub, 2 mul, 3 sq, 4 invert, 5 neg; a, b 256-bit words; out canonical.
+void hc_fe_op(u32 op, const u32 a[8], const u32 b[8], u32 out[8]) {
+    const fe x = fe_from_words(a), y = fe_from_words(b);
+    fe r;
+    switch (op) {
+        case 0: r = fe_add(x, y); break;
+        case 1: r = fe_sub(x, y); break;
+        case 2: r = fe_mul(x, y); break;
+        case 3: r = fe_sq(x); break;
+        case 4: r = fe_invert(x); break;
+        default: r = fe_neg(x); break;
+    }
+    fe_to_words(r, out);
+}
+
+int hc_digits(const u32 s[8], u32 w, i32* out) {
+    mk_digits g = mk_digits_of(s, w);
+    for (u32 i = 0; i < g.rows; i++) out[i] = mk_next_digit(g);
+    return (int)g.rows;
+}
+
+u64 hc_table_words(u32 w) { return (u64)mk_rows(w) * mk_cols(w) * MK_ENTRY_WORDS; }
+
+// The whole comb table of width w, built on the CPU by the kernel's entry code.
+void hc_table_build(u32 w, u32* table) {
+    std::vector<mk_ext> bases(mk_rows(w) + 1);
+    mk_row_bases(w, bases.data());
+    const u32 rows = mk_rows(w), cols = mk_cols(w);
+    for (u32 r = 0; r < rows; r++)
+        for (u32 k = 0; k < cols; k++) mk_table_entry(w, bases.data(), r, k, table + ((u64)r * cols + k) * MK_ENTRY_WORDS);
+}
+
+void hc_table_entry(u32 w, u32 row, u32 k, u32 out[32]) {
+    std::vector<mk_ext> bases(mk_rows(w) + 1);
+    mk_row_bases(w, bases.data());
+    mk_table_entry(w, bases.data(), row, k, out);
+}
+
+void hc_full_pk(const u32* table, u32 w, const u32 s[8], u32 pk[8]) { mk_full_pk(table, w, s, pk); }
+
+// One search lane: keys first .. first+K-1, y words (x sign bit 0) and candidate flags,
+// as the search kernel computes them.
+int hc_lane(const u32* table, u32 w, const u32 key[8], u32 source, u64 first, u32 K, const u8* needle, u32 len,
+            u32 fold, u32 mode, u32* ys, u8* cand) {
+    matcher_host m;
+    if (build_matcher(needle, len, fold, mode, m) != 0) return -1;
+    mk_matcher mt;
+    mt.mode = m.mode;
+    mt.len = m.len;
+    mt.mask = m.mask;
+    mt.dfa = m.dfa.empty() ? nullptr : m.dfa.data();
+    std::vector<u32> spill((size_t)K * 30);
+    mk_spill sp;
+    sp.base = spill.data();
+    sp.stride = 1;
+    mk_lane_tile(table, w, K, key, source, first, sp, [&](u32 j, const u32 pk[8]) {
+        memcpy(ys + 8 * j, pk, 32);
+        cand[j] = mk_pk_match(pk, mt, true) ? 1 : 0;
+    });
+    return 0;
+}
+
+int hc_build_mask(const u8* needle, u32 len, u32 fold, u32 mode, u64 mask[64]) {
+    matcher_host m;
+    if (build_matcher(needle, len, fold, mode, m) != 0) return -1;
+    memcpy(mask, m.mask, sizeof m.mask);
+    return 0;
+}
+
+// Match a full key (sign known unless sign_unknown) against a needle or DFA table.
+int hc_match(const u32 pk[8], const u8* needle, u32 len, u32 fold, u32 mode, u32 sign_unknown) {
+    matcher_host m;
+    if (build_matcher(needle, len, fold, mode, m) != 0) return -1;
+    mk_matcher mt;
+    mt.mode = m.mode;
+    mt.len = m.len;
+    mt.mask = m.mask;
+    mt.dfa = m.dfa.empty() ? nullptr : m.dfa.data();
+    return mk_pk_match(pk, mt, sign_unknown != 0) ? 1 : 0;
+}
+
+}  // extern "C"

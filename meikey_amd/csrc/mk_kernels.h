@@ -1,0 +1,51 @@
+// Device-side data structures shared by the kernels and the host launch code.
+#pragma once
+
+#include "mk_common.h"
+
+// One search: its seed key, how seeds are made and what it matches.
+struct mk_search_desc {
+    u32 key[8];       // ChaCha20 key, or the counter base (little-endian words of the 32 bytes)
+    u32 source;       // 0 = ChaCha20 seeds, 1 = counter seeds (low-security)
+    u32 mode;         // MK_MODE_CONTAINS / SUFFIX / REGEX
+    u32 len;          // needle length, or DFA state count for MK_MODE_REGEX
+    u32 stop_on_hit;  // skip the rest of this search's tiles once it has a candidate
+    u32 mask_off;     // offset of its 64-entry bitap table in the mask pool (u64 units)
+    u32 dfa_off;      // offset of its DFA table in the DFA pool (u16 units)
+    u32 pad[2];
+};
+
+// A contiguous index range of one search, cut into tiles of 64 * keys_per_thread keys.
+// Segments of one launch cover tiles [tile_begin, tile_end) of the launch in order.
+struct mk_seg {
+    u64 first;       // key index of the segment's first tile
+    u64 tile_begin;  // launch-wide tile number where the segment starts
+    u64 tile_end;
+    u32 search;      // index into the search descriptors
+    u32 pad;
+};
+
+// A search-kernel candidate: the key may match (its x sign is not yet known).
+struct mk_cand {
+    u32 search;
+    u32 pad;
+    u64 index;
+};
+
+// An exact hit: the key matches with its real x sign.
+struct mk_hit {
+    u32 search;
+    u32 pad;
+    u64 index;
+    u32 seed[8];
+    u32 pk[8];
+};
+
+// Launch-wide counters, cleared before each step.
+struct mk_counters {
+    u64 queue;      // next tile to hand out
+    u32 n_cands;    // candidates appended (may exceed the buffer; the excess is dropped and flagged)
+    u32 n_hits;     // exact hits appended
+    u32 overflow;   // set when candidates or hits did not fit
+    u32 pad[3];
+};

@@ -1,0 +1,255 @@
+// gfx950 kernels of the vanity-key search: comb-table build, key derivation, the
+// search kernel and the exact-hit confirmation.
+//
+// Search kernel (the hot path). A launch is a list of segments (index ranges of one
+// or more searches) cut into tiles of 64 lanes x K keys. Each workgroup is one
+// wave64; a wave takes the next tile from a launch-wide atomic counter, so searches
+// of uneven length share the chip without a static split, and exits when the tiles
+// run out. Lane l of a tile owns K consecutive key indices, so the ChaCha20 block of
+// an even index also gives the seed of the odd index after it. Per key:
+//   ChaCha20 -> SHA-512 + clamp -> comb multiply (rows - 2 extended+Niels additions and
+//   a 3-multiplication fraction for the last row) -> y numerator/denominator.
+// The K denominators of a lane are inverted together (Montgomery's trick: one
+// inversion, 3 multiplications per key). The numerators, denominators and prefix
+// products wait in a global buffer laid out [key][word][lane] so every access of a
+// wave is one contiguous 256-byte line (HBM traffic 240 B per key, against the
+// 16-20 KFLOP-equivalent of integer work per key). y is compressed with the x sign
+// still unknown and matched with sign_unknown (superset of the true hits); lanes
+// with a candidate append it through one wave ballot and one atomic.
+// The confirm kernel re-derives each candidate with x, matches exactly and writes
+// exact hits (index, seed, public key).
+#include <hip/hip_runtime.h>
+
+#include "mk_field.h"
+#include "mk_hash.h"
+#include "mk_kernels.h"
+#include "mk_match.h"
+#include "mk_point.h"
+#include "mk_tile.h"
+
+// -------------------------------------------------------------- table build
+
+// Row bases and the top row's offset (mk_row_bases). One thread.
+extern "C" __global__ void mk_table_bases_kernel(u32 w, mk_ext* out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    mk_row_bases(w, out);
+}
+
+// One thread per entry, entries [e0, e0 + n) of the row-major table.
+extern "C" __global__ void __launch_bounds__(64) mk_table_fill_kernel(u32 w, const mk_ext* __restrict__ bases,
+                                                                      u64 e0, u64 n, u32* __restrict__ table) {
+    const u64 e = e0 + (u64)blockIdx.x * 64 + threadIdx.x;
+    if (e >= e0 + n) return;
+    const u32 rows = mk_rows(w), cols = mk_cols(w);
+    const u32 row = (u32)(e / cols), k = (u32)(e % cols);
+    if (row >= rows) return;
+    mk_table_entry(w, bases, row, k, table + e * MK_ENTRY_WORDS);
+}
+
+// --------------------------------------------------------------- derivation
+
+// Public keys (and optionally seeds) of indices first .. first + n - 1 of one search
+// key. One thread per key.
+extern "C" __global__ void __launch_bounds__(64) mk_derive_kernel(const u32* __restrict__ table, u32 w,
+                                                                  const u32* __restrict__ key, u32 source,
+                                                                  u64 first, u64 n, u32* __restrict__ out_pk,
+                                                                  u32* __restrict__ out_seed) {
+    const u64 i = (u64)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    u32 k[8], block[16], seed[8], s[8], pk[8];
+    for (int j = 0; j < 8; j++) k[j] = key[j];
+    mk_seed_at(k, source, first + i, block, true, seed);
+    mk_secret_scalar(seed, s);
+    mk_full_pk(table, w, s, pk);
+    for (int j = 0; j < 8; j++) out_pk[i * 8 + j] = pk[j];
+    if (out_seed)
+        for (int j = 0; j < 8; j++) out_seed[i * 8 + j] = seed[j];
+}
+
+// Public keys of explicit 32-byte seeds (known-answer self-check).
+extern "C" __global__ void __launch_bounds__(64) mk_derive_seeds_kernel(const u32* __restrict__ table, u32 w,
+                                                                        const u32* __restrict__ seeds, u32 n,
+                                                                        u32* __restrict__ out_pk) {
+    const u32 i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    u32 seed[8], s[8], pk[8];
+    for (int j = 0; j < 8; j++) seed[j] = seeds[i * 8 + j];
+    mk_secret_scalar(seed, s);
+    mk_full_pk(table, w, s, pk);
+    for (int j = 0; j < 8; j++) out_pk[i * 8 + j] = pk[j];
+}
+
+// ------------------------------------------------------------------- search
+
+__device__ static u64 mk_uniform64(u64 v) {
+    const u32 lo = __builtin_amdgcn_readfirstlane((u32)v);
+    const u32 hi = __builtin_amdgcn_readfirstlane((u32)(v >> 32));
+    return (u64)hi << 32 | lo;
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+    mk_search_kernel(const u32* __restrict__ table, u32 w, u32 K, const mk_search_desc* __restrict__ descs,
+                     const mk_seg* __restrict__ segs, u32 nsegs, u64 total_tiles, const u64* __restrict__ masks,
+                     const u16* __restrict__ dfas, u32* __restrict__ scratch, mk_counters* __restrict__ ctr,
+                     mk_cand* __restrict__ cands, u32 cand_cap, u32* __restrict__ stop) {
+    __shared__ u64 lmask[64];
+    const u32 lane = threadIdx.x;
+    const u64 lanes = (u64)gridDim.x * 64;
+    const u64 gl = (u64)blockIdx.x * 64 + lane;
+    u32 cached = 0xffffffffu;
+    for (;;) {
+        u64 tile = 0;
+        if (lane == 0) tile = atomicAdd((unsigned long long*)&ctr->queue, 1ull);
+        tile = mk_uniform64(tile);
+        if (tile >= total_tiles) break;
+        // segment holding this tile (binary search; segments are few and in order)
+        u32 lo = 0, hi = nsegs;
+        while (hi - lo > 1) {
+            const u32 mid = (lo + hi) >> 1;
+            if (segs[mid].tile_begin <= tile) lo = mid;
+            else hi = mid;
+        }
+        const mk_seg sg = segs[lo];
+        const mk_search_desc& dsc = descs[sg.search];
+        if (dsc.stop_on_hit && __atomic_load_n(&stop[sg.search], __ATOMIC_RELAXED) != 0) continue;
+        if (sg.search != cached) {
+            __syncthreads();
+            lmask[lane] = masks[(u64)dsc.mask_off + lane];
+            __syncthreads();
+            cached = sg.search;
+        }
+        mk_matcher mt;
+        mt.mode = dsc.mode;
+        mt.len = dsc.len;
+        mt.mask = lmask;
+        mt.dfa = dfas + dsc.dfa_off;
+        u32 key[8];
+        MK_UNROLL
+        for (int i = 0; i < 8; i++) key[i] = dsc.key[i];
+        const u64 first = sg.first + (tile - sg.tile_begin) * 64ull * K + (u64)lane * K;
+        const u32 search = sg.search, stop_on_hit = dsc.stop_on_hit;
+        mk_spill sp;
+        sp.base = scratch + gl;
+        sp.stride = lanes;
+        mk_lane_tile(table, w, K, key, dsc.source, first, sp, [&](u32 jj, const u32 pk[8]) {
+            const bool cand = mk_pk_match(pk, mt, true);
+            const u64 bal = __ballot(cand);
+            if (bal == 0) return;
+            const u32 leader = (u32)__builtin_ctzll(bal);
+            const u32 below = __builtin_amdgcn_mbcnt_hi((u32)(bal >> 32), __builtin_amdgcn_mbcnt_lo((u32)bal, 0u));
+            u32 base = 0;
+            if (lane == leader) {
+                base = atomicAdd(&ctr->n_cands, (u32)__builtin_popcountll(bal));
+                if (stop_on_hit) atomicExch(&stop[search], 1u);
+            }
+            base = __shfl(base, (int)leader);
+            if (cand) {
+                const u32 slot = base + below;
+                if (slot < cand_cap) {
+                    cands[slot].search = search;
+                    cands[slot].pad = 0;
+                    cands[slot].index = first + jj;
+                } else {
+                    atomicOr(&ctr->overflow, 1u);
+                }
+            }
+        });
+    }
+}
+
+// Exact check of every candidate: full key with its x sign, matched with the sign
+// known. One thread per candidate slot.
+extern "C" __global__ void __launch_bounds__(64)
+    mk_confirm_kernel(const u32* __restrict__ table, u32 w, const mk_search_desc* __restrict__ descs,
+                      const u64* __restrict__ masks, const u16* __restrict__ dfas, mk_counters* __restrict__ ctr,
+                      const mk_cand* __restrict__ cands, u32 cand_cap, mk_hit* __restrict__ hits, u32 hit_cap) {
+    const u32 i = blockIdx.x * 64 + threadIdx.x;
+    const u32 n = ctr->n_cands < cand_cap ? ctr->n_cands : cand_cap;
+    if (i >= n) return;
+    const mk_cand c = cands[i];
+    const mk_search_desc& dsc = descs[c.search];
+    u32 key[8], block[16], seed[8], s[8], pk[8];
+    for (int j = 0; j < 8; j++) key[j] = dsc.key[j];
+    mk_seed_at(key, dsc.source, c.index, block, true, seed);
+    mk_secret_scalar(seed, s);
+    mk_full_pk(table, w, s, pk);
+    mk_matcher mt;
+    mt.mode = dsc.mode;
+    mt.len = dsc.len;
+    mt.mask = masks + dsc.mask_off;
+    mt.dfa = dfas + dsc.dfa_off;
+    if (!mk_pk_match(pk, mt, false)) return;
+    const u32 slot = atomicAdd(&ctr->n_hits, 1u);
+    if (slot >= hit_cap) {
+        atomicOr(&ctr->overflow, 2u);
+        return;
+    }
+    mk_hit& h = hits[slot];
+    h.search = c.search;
+    h.pad = 0;
+    h.index = c.index;
+    for (int j = 0; j < 8; j++) h.seed[j] = seed[j];
+    for (int j = 0; j < 8; j++) h.pk[j] = pk[j];
+}
+
+// Clears the launch counters and the first nstop stop flags with a kernel (a plain
+// fill rather than a memset node, so the step can be captured in a graph).
+extern "C" __global__ void mk_reset_kernel(mk_counters* ctr, u32* stop, u32 nstop) {
+    const u32 i = blockIdx.x * 64 + threadIdx.x;
+    if (i == 0) {
+        ctr->queue = 0;
+        ctr->n_cands = 0;
+        ctr->n_hits = 0;
+        ctr->overflow = 0;
+    }
+    if (stop != nullptr && i < nstop) stop[i] = 0;
+}
+
+// ------------------------------------------------------------ host launchers
+
+#include "mk_launch.h"
+
+hipError_t mk_k_table_bases(hipStream_t s, u32 w, void* bases) {
+    hipLaunchKernelGGL(mk_table_bases_kernel, dim3(1), dim3(64), 0, s, w, (mk_ext*)bases);
+    return hipGetLastError();
+}
+
+hipError_t mk_k_table_fill(hipStream_t s, u32 w, const void* bases, u64 e0, u64 n, u32* table) {
+    const u64 blocks = (n + 63) / 64;
+    hipLaunchKernelGGL(mk_table_fill_kernel, dim3((u32)blocks), dim3(64), 0, s, w, (const mk_ext*)bases, e0, n, table);
+    return hipGetLastError();
+}
+
+hipError_t mk_k_derive(hipStream_t s, const u32* table, u32 w, const u32* key, u32 source, u64 first, u64 n,
+                       u32* out_pk, u32* out_seed) {
+    const u64 blocks = (n + 63) / 64;
+    hipLaunchKernelGGL(mk_derive_kernel, dim3((u32)blocks), dim3(64), 0, s, table, w, key, source, first, n, out_pk,
+                       out_seed);
+    return hipGetLastError();
+}
+
+hipError_t mk_k_derive_seeds(hipStream_t s, const u32* table, u32 w, const u32* seeds, u32 n, u32* out_pk) {
+    hipLaunchKernelGGL(mk_derive_seeds_kernel, dim3((n + 63) / 64), dim3(64), 0, s, table, w, seeds, n, out_pk);
+    return hipGetLastError();
+}
+
+hipError_t mk_k_reset(hipStream_t s, mk_counters* ctr, u32* stop, u32 nstop) {
+    hipLaunchKernelGGL(mk_reset_kernel, dim3(nstop > 64 ? (nstop + 63) / 64 : 1), dim3(64), 0, s, ctr, stop, nstop);
+    return hipGetLastError();
+}
+
+hipError_t mk_k_search(hipStream_t s, u32 grid, const u32* table, u32 w, u32 K, const mk_search_desc* descs,
+                       const mk_seg* segs, u32 nsegs, u64 total_tiles, const u64* masks, const u16* dfas,
+                       u32* scratch, mk_counters* ctr, mk_cand* cands, u32 cand_cap, u32* stop) {
+    hipLaunchKernelGGL(mk_search_kernel, dim3(grid), dim3(64), 0, s, table, w, K, descs, segs, nsegs, total_tiles,
+                       masks, dfas, scratch, ctr, cands, cand_cap, stop);
+    return hipGetLastError();
+}
+
+hipError_t mk_k_confirm(hipStream_t s, const u32* table, u32 w, const mk_search_desc* descs, const u64* masks,
+                        const u16* dfas, mk_counters* ctr, const mk_cand* cands, u32 cand_cap, mk_hit* hits,
+                        u32 hit_cap) {
+    hipLaunchKernelGGL(mk_confirm_kernel, dim3((cand_cap + 63) / 64), dim3(64), 0, s, table, w, descs, masks, dfas,
+                       ctr, cands, cand_cap, hits, hit_cap);
+    return hipGetLastError();
+}

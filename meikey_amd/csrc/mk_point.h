@@ -1,0 +1,283 @@
+// Points of edwards25519 (-x^2 + y^2 = 1 + d x^2 y^2) and the fixed-base comb
+// multiply s*B used for every candidate key.
+//
+// The clamped scalar is s = 8 * (2^251 + m). The comb table of width w has
+// rows = ceil(251 / w) rows; entry (i, k) is k * 2^(w*i) * 8B as an affine point in
+// "Niels" form (y+x, y-x, 2dxy), and the top row also adds 2^254 * B, so that
+// s*B = sum_i d_i * entry(i, |d_i|) for the signed base-2^w digits d_i of m
+// (reference src/service/derive/comb.rs:1-8, 289-339, 414-445). Row 0 stores 2xy in
+// place of 2dxy so the first point becomes extended coordinates with no multiply
+// (X = 2x, Y = 2y, Z = 2, T = 2xy; reference src/service/gpu.rs:1466-1470).
+#pragma once
+
+#include "mk_field.h"
+
+struct mk_ext {
+    fe x, y, z, t;
+};
+
+struct mk_niels {
+    fe ypx, ymx, xy2d;
+};
+
+// Extended + affine Niels point, 7 multiplications (madd for a = -1 with the
+// 2d factor folded into the table). Inputs: p carried, q carried.
+MK_DEV mk_ext mk_add_niels(const mk_ext& p, const mk_niels& q) {
+    const fe pp = fe_mul(fe_add(p.y, p.x), q.ypx);
+    const fe mm = fe_mul(fe_sub(p.y, p.x), q.ymx);
+    const fe txy = fe_mul(p.t, q.xy2d);
+    const fe z2 = fe_add(p.z, p.z);
+    const fe x = fe_sub(pp, mm), y = fe_add(pp, mm);
+    const fe z = fe_add(z2, txy), t = fe_sub(z2, txy);
+    mk_ext r;
+    r.x = fe_mul(t, x);  // t (limbs < 2^28) only ever on the first operand
+    r.y = fe_mul(y, z);
+    r.z = fe_mul(t, z);
+    r.t = fe_mul(x, y);
+    return r;
+}
+
+// The last addition as fractions: y = (pp + mm) / (2Z - T*2dxy) and
+// x = (pp - mm) / (2Z + T*2dxy), 3 multiplications (reference kernel.cu:1613-1627).
+// Denominators are returned carried.
+MK_DEV void mk_add_niels_frac(const mk_ext& p, const mk_niels& q, fe& yn, fe& yd, fe& xn, fe& xd) {
+    const fe pp = fe_mul(fe_add(p.y, p.x), q.ypx);
+    const fe mm = fe_mul(fe_sub(p.y, p.x), q.ymx);
+    const fe txy = fe_mul(p.t, q.xy2d);
+    const fe z2 = fe_add(p.z, p.z);
+    yn = fe_add(pp, mm);
+    yd = fe_carry(fe_sub(z2, txy));
+    xn = fe_sub(pp, mm);
+    xd = fe_carry(fe_add(z2, txy));
+}
+
+// Row 0's entry (third field 2xy) to extended coordinates with no multiplication.
+MK_DEV mk_ext mk_row0_extended(const mk_niels& n) {
+    mk_ext p;
+    p.x = fe_carry(fe_sub(n.ypx, n.ymx));
+    p.y = fe_carry(fe_add(n.ypx, n.ymx));
+    p.z = fe_small(2);
+    p.t = n.xy2d;
+    return p;
+}
+
+// Signed digits of m, produced one per row from low to high. m is bits 3..253 of the
+// clamped scalar; each digit is the next w bits plus the carry, mapped to
+// [-2^(w-1), 2^(w-1)) with a carry out, except the top digit, which keeps the carry
+// and lies in [0, 2^(w-1)] (comb.rs:410-445).
+struct mk_digits {
+    u64 m[4];
+    u32 w, rows, i;
+    i32 carry;
+};
+
+MK_DEV mk_digits mk_digits_of(const u32 s[8], u32 w) {
+    mk_digits g;
+    u64 x[4];
+    MK_UNROLL
+    for (int i = 0; i < 4; i++) x[i] = (u64)s[2 * i + 1] << 32 | s[2 * i];
+    g.m[0] = x[0] >> 3 | x[1] << 61;
+    g.m[1] = x[1] >> 3 | x[2] << 61;
+    g.m[2] = x[2] >> 3 | x[3] << 61;
+    g.m[3] = (x[3] >> 3) & ((1ull << (MK_M_BITS - 192)) - 1);
+    g.w = w;
+    g.rows = mk_rows(w);
+    g.i = 0;
+    g.carry = 0;
+    return g;
+}
+
+MK_DEV u64 mk_digit_word(const mk_digits& g, u32 q) {
+    return q == 0 ? g.m[0] : q == 1 ? g.m[1] : q == 2 ? g.m[2] : q == 3 ? g.m[3] : 0ull;
+}
+
+MK_DEV i32 mk_next_digit(mk_digits& g) {
+    const u32 off = g.i * g.w, q = off >> 6, r = off & 63;
+    u64 bits = mk_digit_word(g, q) >> r;
+    if (r != 0) bits |= mk_digit_word(g, q + 1) << (64 - r);
+    const i32 v = (i32)(bits & ((1ull << g.w) - 1)) + g.carry;
+    i32 d = v;
+    if (g.i + 1 < g.rows) {
+        g.carry = (v + (1 << (g.w - 1))) >> g.w;
+        d = v - (g.carry << g.w);
+    }
+    g.i++;
+    return d;
+}
+
+// Raw entry words as loaded from the table (30 limbs + 2 pad words).
+struct mk_entry {
+    u32 w[MK_ENTRY_WORDS];
+};
+
+#if defined(__HIPCC__)
+// One 128-byte entry: eight 16-byte vector loads of one cache line.
+MK_DEV void mk_load_entry(const u32* __restrict__ table, u64 row, i32 d, u32 cols, mk_entry& e) {
+    const u32 k = (u32)(d < 0 ? -d : d);
+    const uint4* p = (const uint4*)(table + (row * cols + k) * MK_ENTRY_WORDS);
+    MK_UNROLL
+    for (int q = 0; q < MK_ENTRY_WORDS / 4; q++) {
+        const uint4 v = p[q];
+        e.w[4 * q] = v.x;
+        e.w[4 * q + 1] = v.y;
+        e.w[4 * q + 2] = v.z;
+        e.w[4 * q + 3] = v.w;
+    }
+}
+#else
+MK_DEV void mk_load_entry(const u32* table, u64 row, i32 d, u32 cols, mk_entry& e) {
+    const u32 k = (u32)(d < 0 ? -d : d);
+    const u32* p = table + (row * cols + k) * MK_ENTRY_WORDS;
+    for (int i = 0; i < MK_ENTRY_WORDS; i++) e.w[i] = p[i];
+}
+#endif
+
+// Entry words to a Niels point, negated for a negative digit (-P = (-x, y): swap y+x
+// and y-x, negate the third field).
+MK_DEV mk_niels mk_entry_niels(const mk_entry& e, i32 d) {
+    mk_niels n;
+    MK_UNROLL
+    for (int i = 0; i < 10; i++) {
+        n.ypx.v[i] = e.w[i];
+        n.ymx.v[i] = e.w[10 + i];
+        n.xy2d.v[i] = e.w[20 + i];
+    }
+    if (d < 0) {
+        const fe t = n.ypx;
+        n.ypx = n.ymx;
+        n.ymx = t;
+        n.xy2d = fe_neg(n.xy2d);
+    }
+    return n;
+}
+
+// s*B as fractions yn/yd (and xn/xd) from the comb table of width w. The entry of the
+// next row is loaded before the current row is added, so its latency overlaps the
+// seven multiplications of the addition.
+MK_DEV void mk_mul_base_frac(const u32* __restrict__ table, u32 w, const u32 s[8], fe& yn, fe& yd, fe& xn,
+                             fe& xd) {
+    mk_digits g = mk_digits_of(s, w);
+    const u32 rows = g.rows, cols = mk_cols(w);
+    i32 d = mk_next_digit(g);
+    mk_entry cur, next;
+    mk_load_entry(table, 0, d, cols, cur);
+    i32 dn = mk_next_digit(g);
+    mk_load_entry(table, 1, dn, cols, next);
+    mk_ext acc = mk_row0_extended(mk_entry_niels(cur, d));
+    MK_NO_UNROLL
+    for (u32 row = 1; row + 1 < rows; row++) {
+        cur = next;
+        d = dn;
+        dn = mk_next_digit(g);
+        mk_load_entry(table, row + 1, dn, cols, next);
+        acc = mk_add_niels(acc, mk_entry_niels(cur, d));
+    }
+    mk_add_niels_frac(acc, mk_entry_niels(next, dn), yn, yd, xn, xd);
+}
+
+// Full compressed public key (y with the x sign) of one scalar; one inversion.
+MK_DEV void mk_full_pk(const u32* __restrict__ table, u32 w, const u32 s[8], u32 pk[8]) {
+    fe yn, yd, xn, xd;
+    mk_mul_base_frac(table, w, s, yn, yd, xn, xd);
+    const fe inv = fe_invert(fe_mul(yd, xd));
+    const fe y = fe_mul(fe_mul(yn, xd), inv);
+    const fe x = fe_mul(fe_mul(xn, yd), inv);
+    u32 xw[8];
+    fe_to_words(y, pk);
+    fe_to_words(x, xw);
+    pk[7] |= (xw[0] & 1u) << 31;
+}
+
+// ------------------------------------------------------------------ constants
+
+// Base point B and 2d as little-endian words (RFC 8032 s5.1).
+MK_CONST u32 MK_BX[8] = {0x8f25d51au, 0xc9562d60u, 0x9525a7b2u, 0x692cc760u,
+                         0xfdd6dc5cu, 0xc0a4e231u, 0xcd6e53feu, 0x216936d3u};
+MK_CONST u32 MK_BY[8] = {0x66666658u, 0x66666666u, 0x66666666u, 0x66666666u,
+                         0x66666666u, 0x66666666u, 0x66666666u, 0x66666666u};
+MK_CONST u32 MK_D2[8] = {0x26b2f159u, 0xebd69b94u, 0x8283b156u, 0x00e0149au,
+                         0xeef3d130u, 0x198e80f2u, 0x56dffce7u, 0x2406d9dcu};
+
+MK_DEV fe mk_const(const u32* w) {
+    u32 t[8];
+    MK_UNROLL
+    for (int i = 0; i < 8; i++) t[i] = w[i];
+    return fe_from_words(t);
+}
+
+// Unified addition in extended coordinates (RFC 8032 s5.1.4), complete on
+// edwards25519, used for doubling as well. Table build only.
+MK_DEV mk_ext mk_ext_add(const mk_ext& p, const mk_ext& q, const fe& d2) {
+    const fe a = fe_mul(fe_sub(p.y, p.x), fe_sub(q.y, q.x));
+    const fe b = fe_mul(fe_add(p.y, p.x), fe_add(q.y, q.x));
+    const fe c = fe_mul(fe_mul(p.t, d2), q.t);
+    const fe d = fe_mul(fe_add(p.z, p.z), q.z);
+    const fe e = fe_sub(b, a), f = fe_sub(d, c), g = fe_add(d, c), h = fe_add(b, a);
+    mk_ext r;
+    r.x = fe_mul(e, f);
+    r.y = fe_mul(g, h);
+    r.t = fe_mul(e, h);
+    r.z = fe_mul(f, g);
+    return r;
+}
+
+MK_DEV mk_ext mk_ext_identity() {
+    mk_ext p;
+    p.x = fe_small(0);
+    p.y = fe_small(1);
+    p.z = fe_small(1);
+    p.t = fe_small(0);
+    return p;
+}
+
+// Row bases 2^(w*i) * 8B for every row, plus the top row's offset 2^254 * B, as
+// extended points: out[i] for i < rows, out[rows] = the offset.
+MK_DEV void mk_row_bases(u32 w, mk_ext* out) {
+    const fe d2 = mk_const(MK_D2);
+    mk_ext p;
+    p.x = mk_const(MK_BX);
+    p.y = mk_const(MK_BY);
+    p.z = fe_small(1);
+    p.t = fe_mul(p.x, p.y);
+    for (int i = 0; i < 3; i++) p = mk_ext_add(p, p, d2);
+    const u32 rows = mk_rows(w);
+    for (u32 i = 0; i < rows; i++) {
+        out[i] = p;
+        if (i + 1 == rows) {
+            // 2^c times the top row's base is 2^254 * B, c = 251 - w * (rows - 1)
+            mk_ext t = p;
+            for (u32 c = 0; c < MK_M_BITS - w * (rows - 1); c++) t = mk_ext_add(t, t, d2);
+            out[rows] = t;
+        }
+        for (u32 c = 0; c < w; c++) p = mk_ext_add(p, p, d2);
+    }
+}
+
+// Entry (row, k) = k * base_row (+ 2^254 * B on the top row) as (y+x, y-x, 2dxy) in
+// canonical limbs, row 0 with 2xy as its third field; 32 words (2 pad words zero).
+MK_DEV void mk_table_entry(u32 w, const mk_ext* bases, u32 row, u32 k, u32* out) {
+    const u32 rows = mk_rows(w);
+    const fe d2 = mk_const(MK_D2);
+    const mk_ext base = bases[row];
+    mk_ext m = mk_ext_identity();
+    for (int b = (int)w - 1; b >= 0; b--) {
+        m = mk_ext_add(m, m, d2);
+        if ((k >> b) & 1u) m = mk_ext_add(m, base, d2);
+    }
+    const mk_ext acc = row + 1 == rows ? mk_ext_add(bases[rows], m, d2) : m;
+    const fe zi = fe_invert(acc.z);
+    const fe x = fe_mul(acc.x, zi), y = fe_mul(acc.y, zi);
+    const fe xy = fe_mul(x, y);
+    fe parts[3];
+    parts[0] = fe_carry(fe_add(y, x));
+    parts[1] = fe_carry(fe_sub(y, x));
+    parts[2] = row == 0 ? fe_carry(fe_add(xy, xy)) : fe_mul(xy, d2);
+    for (int f = 0; f < 3; f++) {
+        u32 wds[8];
+        fe_to_words(parts[f], wds);
+        const fe c = fe_from_words(wds);  // canonical limbs
+        for (int i = 0; i < 10; i++) out[10 * f + i] = c.v[i];
+    }
+    out[30] = 0;
+    out[31] = 0;
+}

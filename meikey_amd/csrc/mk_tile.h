@@ -1,0 +1,64 @@
+// One lane's share of a search tile: K consecutive key indices, their y coordinates
+// from a single field inversion, handed one by one (last to first) to a callback.
+// The search kernel and the CPU test build both run this exact code.
+#pragma once
+
+#include "mk_hash.h"
+#include "mk_point.h"
+
+// Per-lane spill area for the K numerators, denominators and prefix products.
+// Element `i` of field slot `slot` lives at base[(slot * 10 + i) * stride]; with
+// stride = number of lanes and base offset by the lane, the words of one slot are
+// contiguous across a wave.
+struct mk_spill {
+    u32* base;
+    u64 stride;
+    MK_MEMBER void store(u32 slot, const fe& a) const {
+        u32* p = base + (u64)slot * 10 * stride;
+        MK_UNROLL
+        for (int i = 0; i < 10; i++) p[(u64)i * stride] = a.v[i];
+    }
+    MK_MEMBER fe load(u32 slot) const {
+        const u32* p = base + (u64)slot * 10 * stride;
+        fe a;
+        MK_UNROLL
+        for (int i = 0; i < 10; i++) a.v[i] = p[(u64)i * stride];
+        return a;
+    }
+};
+
+// Keys first .. first + K - 1: seed -> scalar -> (yn, yd); one inversion of the
+// product of the yd; then, from the last key to the first, y = yn / yd is compressed
+// (x sign bit left 0) and on_key(j, pk_words) is called.
+template <class F>
+MK_DEV void mk_lane_tile(const u32* __restrict__ table, u32 w, u32 K, const u32 key[8], u32 source, u64 first,
+                         const mk_spill& sp, F on_key) {
+    u32 block[16];
+    fe pre;
+    MK_NO_UNROLL
+    for (u32 j = 0; j < K; j++) {
+        u32 seed[8], s[8];
+        mk_seed_at(key, source, first + j, block, j == 0, seed);
+        mk_secret_scalar(seed, s);
+        fe yn, yd, xn, xd;
+        mk_mul_base_frac(table, w, s, yn, yd, xn, xd);
+        pre = j == 0 ? yd : fe_mul(pre, yd);
+        sp.store(3 * j, yn);
+        sp.store(3 * j + 1, yd);
+        sp.store(3 * j + 2, pre);
+    }
+    fe inv = fe_invert(pre);
+    // 1/yd_j = inv * pre_(j-1) with inv = 1/pre_j; then inv <- inv * yd_j = 1/pre_(j-1)
+    MK_NO_UNROLL
+    for (u32 jj = K; jj-- > 0;) {
+        const fe yn = sp.load(3 * jj);
+        fe zinv = inv;
+        if (jj > 0) {
+            zinv = fe_mul(inv, sp.load(3 * (jj - 1) + 2));
+            inv = fe_mul(inv, sp.load(3 * jj + 1));
+        }
+        u32 pk[8];
+        fe_to_words(fe_mul(yn, zinv), pk);
+        on_key(jj, pk);
+    }
+}

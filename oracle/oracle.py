@@ -1,0 +1,160 @@
+"""ctypes view of the CPU oracle (liboracle.so) — test infrastructure only.
+
+Only tests/, the smoke check and the benchmark's CPU-baseline leg import this
+module. It restates the reference's per-key algorithm on the CPU; the functions
+and their reference citations are in meikey_oracle.c.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF = os.path.join(HERE, "_ref", "libmeikey_ref.so")
+
+CONTAINS, SUFFIX, REGEX = 0, 1, 2
+
+
+def build(ref: bool = True) -> None:
+    """Compile liboracle.so, and the reference host build where its sources exist."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if ref and os.path.exists("/root/reference/src/service/gpu/host.cpp"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build(ref=False)
+        L = ctypes.CDLL(LIB)
+        u8p, u64 = ctypes.c_char_p, ctypes.c_uint64
+        L.orc_seed.argtypes = [u8p, ctypes.c_int, u64, ctypes.c_void_p]
+        L.orc_chacha20_block.argtypes = [ctypes.c_void_p, u64, ctypes.c_void_p]
+        L.orc_sha512.argtypes = [u8p, ctypes.c_size_t, ctypes.c_void_p]
+        L.orc_secret_scalar.argtypes = [u8p, ctypes.c_void_p]
+        L.orc_fe_op.argtypes = [ctypes.c_int, u8p, u8p, ctypes.c_void_p]
+        L.orc_digits.argtypes = [u8p, ctypes.c_int, ctypes.c_void_p]
+        L.orc_mul_base_comb.argtypes = [u8p, ctypes.c_int, ctypes.c_void_p]
+        L.orc_mul_base_ladder.argtypes = [u8p, ctypes.c_void_p]
+        L.orc_comb_entry.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p]
+        L.orc_comb_entry_niels.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p]
+        L.orc_public_key.argtypes = [u8p, ctypes.c_void_p]
+        L.orc_public_b64.argtypes = [u8p, ctypes.c_void_p]
+        L.orc_derive.argtypes = [u8p, ctypes.c_int, u64, u64, ctypes.c_void_p, ctypes.c_int]
+        L.orc_match.argtypes = [u8p, u8p, ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
+        L.orc_match_unsigned.argtypes = [u8p, u8p, ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
+        L.orc_search.argtypes = [u8p, ctypes.c_int, u64, u64, u8p, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_void_p, u64, ctypes.c_int]
+        L.orc_search.restype = u64
+        _lib = L
+    return _lib
+
+
+def _buf(n: int):
+    return ctypes.create_string_buffer(n)
+
+
+def seed(key: bytes, source: int, index: int) -> bytes:
+    out = _buf(32)
+    lib().orc_seed(key, source, index, out)
+    return out.raw
+
+
+def chacha20_block(key_words, counter: int) -> list:
+    k = (ctypes.c_uint32 * 8)(*key_words)
+    out = (ctypes.c_uint32 * 16)()
+    lib().orc_chacha20_block(k, counter, out)
+    return list(out)
+
+
+def sha512(msg: bytes) -> bytes:
+    out = _buf(64)
+    lib().orc_sha512(msg, len(msg), out)
+    return out.raw
+
+
+def secret_scalar(seed_: bytes) -> bytes:
+    out = _buf(32)
+    lib().orc_secret_scalar(seed_, out)
+    return out.raw
+
+
+def fe_op(op: int, a: int, b: int) -> int:
+    out = _buf(32)
+    lib().orc_fe_op(op, a.to_bytes(32, "little"), b.to_bytes(32, "little"), out)
+    return int.from_bytes(out.raw, "little")
+
+
+def digits(s: bytes, w: int) -> list:
+    out = (ctypes.c_int32 * 130)()
+    rows = lib().orc_digits(s, w, out)
+    return list(out)[:rows]
+
+
+def mul_base_comb(s: bytes, w: int) -> bytes:
+    out = _buf(32)
+    assert lib().orc_mul_base_comb(s, w, out) == 0
+    return out.raw
+
+
+def mul_base_ladder(s: bytes) -> bytes:
+    out = _buf(32)
+    lib().orc_mul_base_ladder(s, out)
+    return out.raw
+
+
+def comb_entry(w: int, row: int, k: int) -> bytes:
+    out = _buf(32)
+    assert lib().orc_comb_entry(w, row, k, out) == 0
+    return out.raw
+
+
+def comb_entry_niels(w: int, row: int, k: int) -> bytes:
+    out = _buf(96)
+    assert lib().orc_comb_entry_niels(w, row, k, out) == 0
+    return out.raw
+
+
+def public_key(seed_: bytes) -> bytes:
+    out = _buf(32)
+    lib().orc_public_key(seed_, out)
+    return out.raw
+
+
+def public_b64(pk: bytes) -> str:
+    out = _buf(68)
+    lib().orc_public_b64(pk, out)
+    return out.raw.decode()
+
+
+def derive(key: bytes, source: int, first: int, n: int, threads: int = 0) -> bytes:
+    out = _buf(32 * n) if n else _buf(1)
+    lib().orc_derive(key, source, first, n, out, threads)
+    return out.raw[: 32 * n]
+
+
+def match(b64: str, needle: bytes, case_sensitive: bool, mode: int) -> bool:
+    n = len(needle) // 130 if mode == REGEX else len(needle)
+    return bool(lib().orc_match(b64.encode(), needle, n, int(case_sensitive), mode))
+
+
+def match_unsigned(pk: bytes, needle: bytes, case_sensitive: bool, mode: int) -> bool:
+    n = len(needle) // 130 if mode == REGEX else len(needle)
+    return bool(lib().orc_match_unsigned(pk, needle, n, int(case_sensitive), mode))
+
+
+def search(key: bytes, source: int, first: int, n: int, needle: bytes, case_sensitive: bool, mode: int,
+           cap: int = 1 << 20, threads: int = 0) -> list:
+    """Exact hit indices of [first, first + n), ascending."""
+    length = len(needle) // 130 if mode == REGEX else len(needle)
+    hits = (ctypes.c_uint64 * max(cap, 1))()
+    total = lib().orc_search(key, source, first, n, needle, length, int(case_sensitive), mode, hits, cap, threads)
+    if total > cap:
+        raise OverflowError(f"{total} hits exceed cap {cap}")
+    return list(hits)[:total]

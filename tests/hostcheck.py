@@ -1,0 +1,147 @@
+"""ctypes view of libmeikey_hostcheck.so: the kernel arithmetic headers compiled for the
+host (g++), so CPU tests can exercise the exact code the gfx950 kernels run."""
+from __future__ import annotations
+
+import ctypes
+import os
+import struct
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PATH = os.path.join(ROOT, "meikey_amd", "libmeikey_hostcheck.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(PATH):
+            from meikey_amd import build
+            build.build_hostcheck()
+        L = ctypes.CDLL(PATH)
+        P, U32, U64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
+        L.hc_seed.argtypes = [P, U32, U64, P]
+        L.hc_secret_scalar.argtypes = [P, P]
+        L.hc_fe_mul_limbs.argtypes = [P, P, P]
+        L.hc_fe_sq_limbs.argtypes = [P, P]
+        L.hc_fe_to_words.argtypes = [P, P]
+        L.hc_fe_from_words.argtypes = [P, P]
+        L.hc_fe_op.argtypes = [U32, P, P, P]
+        L.hc_digits.argtypes = [P, U32, P]
+        L.hc_table_words.argtypes = [U32]
+        L.hc_table_words.restype = U64
+        L.hc_table_build.argtypes = [U32, P]
+        L.hc_table_entry.argtypes = [U32, U32, U32, P]
+        L.hc_full_pk.argtypes = [P, U32, P, P]
+        L.hc_lane.argtypes = [P, U32, P, U32, U64, U32, ctypes.c_char_p, U32, U32, U32, P, P]
+        L.hc_build_mask.argtypes = [ctypes.c_char_p, U32, U32, U32, P]
+        L.hc_match.argtypes = [P, ctypes.c_char_p, U32, U32, U32, U32]
+        _lib = L
+    return _lib
+
+
+def words(b: bytes):
+    return (ctypes.c_uint32 * 8)(*struct.unpack("<8I", b))
+
+
+def wbytes(w) -> bytes:
+    return struct.pack("<8I", *list(w)[:8])
+
+
+def seed(key: bytes, source: int, index: int) -> bytes:
+    out = (ctypes.c_uint32 * 8)()
+    lib().hc_seed(words(key), source, index, out)
+    return wbytes(out)
+
+
+def secret_scalar(seed_: bytes) -> bytes:
+    out = (ctypes.c_uint32 * 8)()
+    lib().hc_secret_scalar(words(seed_), out)
+    return wbytes(out)
+
+
+def limbs_of(x: int):
+    """Integer to ten limbs at offsets 0,26,51,... (no reduction, x < 2^255)."""
+    offs = [0, 26, 51, 77, 102, 128, 153, 179, 204, 230, 255]
+    return [(x >> offs[i]) & ((1 << (offs[i + 1] - offs[i])) - 1) for i in range(10)]
+
+
+def value_of(limbs) -> int:
+    offs = [0, 26, 51, 77, 102, 128, 153, 179, 204, 230]
+    return sum(int(l) << o for l, o in zip(limbs, offs))
+
+
+def fe_mul_limbs(f, g):
+    a = (ctypes.c_uint32 * 10)(*f)
+    b = (ctypes.c_uint32 * 10)(*g)
+    out = (ctypes.c_uint32 * 10)()
+    lib().hc_fe_mul_limbs(a, b, out)
+    return list(out)
+
+
+def fe_sq_limbs(f):
+    a = (ctypes.c_uint32 * 10)(*f)
+    out = (ctypes.c_uint32 * 10)()
+    lib().hc_fe_sq_limbs(a, out)
+    return list(out)
+
+
+def fe_to_int(limbs) -> int:
+    a = (ctypes.c_uint32 * 10)(*limbs)
+    out = (ctypes.c_uint32 * 8)()
+    lib().hc_fe_to_words(a, out)
+    return int.from_bytes(wbytes(out), "little")
+
+
+def fe_op(op: int, a: int, b: int) -> int:
+    out = (ctypes.c_uint32 * 8)()
+    lib().hc_fe_op(op, words(a.to_bytes(32, "little")), words(b.to_bytes(32, "little")), out)
+    return int.from_bytes(wbytes(out), "little")
+
+
+def digits(s: bytes, w: int):
+    out = (ctypes.c_int32 * 130)()
+    rows = lib().hc_digits(words(s), w, out)
+    return list(out)[:rows]
+
+
+def table(w: int):
+    n = lib().hc_table_words(w)
+    buf = (ctypes.c_uint32 * n)()
+    lib().hc_table_build(w, buf)
+    return buf
+
+
+def table_entry(w: int, row: int, k: int):
+    out = (ctypes.c_uint32 * 32)()
+    lib().hc_table_entry(w, row, k, out)
+    return list(out)
+
+
+def full_pk(tab, w: int, s: bytes) -> bytes:
+    out = (ctypes.c_uint32 * 8)()
+    lib().hc_full_pk(tab, w, words(s), out)
+    return wbytes(out)
+
+
+def lane(tab, w: int, key: bytes, source: int, first: int, K: int, needle: bytes, fold: bool, mode: int):
+    n = len(needle) // 130 if mode == 2 else len(needle)
+    ys = (ctypes.c_uint32 * (8 * K))()
+    cand = (ctypes.c_uint8 * K)()
+    rc = lib().hc_lane(tab, w, words(key), source, first, K, needle, n, int(fold), mode, ys, cand)
+    assert rc == 0
+    return [struct.pack("<8I", *ys[8 * j: 8 * j + 8]) for j in range(K)], list(cand)
+
+
+def build_mask(needle: bytes, fold: bool, mode: int = 0):
+    out = (ctypes.c_uint64 * 64)()
+    rc = lib().hc_build_mask(needle, len(needle), int(fold), mode, out)
+    assert rc == 0
+    return list(out)
+
+
+def match(pk: bytes, needle: bytes, fold: bool, mode: int, sign_unknown: bool) -> bool:
+    n = len(needle) // 130 if mode == 2 else len(needle)
+    rc = lib().hc_match(words(pk), needle, n, int(fold), mode, int(sign_unknown))
+    assert rc >= 0
+    return bool(rc)

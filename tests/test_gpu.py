@@ -1,0 +1,200 @@
+"""Parity of the gfx950 kernels with the CPU oracle, through the C ABI.
+
+Every comparison is bit-exact: public keys (x sign included), table entries and full
+hit sets. Sizes are chosen so the oracle finishes in seconds; the 2^24 config-1 range
+is compared in full.
+"""
+import json
+import os
+import random
+import sys
+
+import pytest
+
+import meikey_amd as M
+from tests import textbook as T
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "golden.json")))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    say("open gpu W=16 batch=32")
+    g = M.Gpu(0, width=16, batch=32)
+    yield g
+    g.close()
+
+
+@pytest.fixture(scope="module")
+def gpu_small():
+    say("open gpu W=12 batch=4 blocks=1")
+    g = M.Gpu(0, width=12, batch=4, blocks=1)
+    yield g
+    g.close()
+
+
+def test_rfc8032_known_answers(gpu):
+    from tests.test_oracle import rfc8032_vectors
+    vecs = rfc8032_vectors()
+    seeds = b"".join(v[1] for v in vecs)
+    pks = gpu.derive_seeds(seeds)
+    assert [pks[32 * i:32 * i + 32] for i in range(len(vecs))] == [v[2] for v in vecs]
+
+
+def test_derive_golden(gpu):
+    for case in GOLDEN["derive"]:
+        key = bytes.fromhex(case["key"])
+        for idx, pk in case["pks"]:
+            assert gpu.derive(key, case["source"], idx, 1) == bytes.fromhex(pk), (case["source"], idx)
+
+
+@pytest.mark.parametrize("source", [0, 1])
+def test_derive_bulk_vs_oracle(gpu, oracle, source):
+    key = random.Random(40 + source).randbytes(32)
+    n = 1 << 18
+    first = (1 << 33) - 1000 if source == 0 else 12345
+    say(f"derive {n} keys source={source}")
+    got, seeds = gpu.derive(key, source, first, n, seeds=True)
+    want = oracle.derive(key, source, first, n)
+    assert got == want
+    for i in (0, 1, n - 1):
+        assert seeds[32 * i:32 * i + 32] == oracle.seed(key, source, first + i)
+
+
+def test_table_entries_vs_oracle(gpu, oracle):
+    from tests.hostcheck import value_of
+    w = 16
+    rows, cols = 16, (1 << 15) + 1
+    rng = random.Random(41)
+    picks = [(0, 0), (0, 1), (rows - 1, 0), (rows - 1, cols - 1), (7, cols - 1)]
+    picks += [(rng.randrange(rows), rng.randrange(cols)) for _ in range(20)]
+    for row, k in picks:
+        words = gpu.table_words(row, k)
+        n = oracle.comb_entry_niels(w, row, k)
+        assert value_of(words[0:10]) == int.from_bytes(n[0:32], "little"), (row, k)
+        assert value_of(words[10:20]) == int.from_bytes(n[32:64], "little"), (row, k)
+        third = value_of(words[20:30])
+        xy2d = int.from_bytes(n[64:96], "little")
+        assert third == (xy2d * T.inv(T.D) % T.P if row == 0 else xy2d), (row, k)
+
+
+def test_search_golden(gpu):
+    for case in GOLDEN["search"]:
+        key = bytes.fromhex(case["key"])
+        hits, tries = gpu.batch_run([{
+            "key": key, "seed_source": case["source"], "needle": case["needle"].encode(),
+            "fold": not case["case_sensitive"], "mode": case["mode"], "first": case["first"],
+            "count": case["count"]}])
+        end = case["first"] + case["count"]
+        got = [h.index for h in hits if h.index < end]
+        assert got == case["hits"]
+
+
+def _spec(key, source, needle, cs, mode, first, count, **kw):
+    d = {"key": key, "seed_source": source, "needle": needle, "fold": not cs, "mode": mode, "first": first,
+         "count": count}
+    d.update(kw)
+    return d
+
+
+@pytest.mark.parametrize("source,needle,cs,mode", [
+    (0, b"ab", False, M.MODE_CONTAINS),
+    (0, b"Q9", True, M.MODE_SUFFIX),
+    (1, b"k+", False, M.MODE_CONTAINS),
+    (0, b"z", True, M.MODE_SUFFIX),
+])
+def test_search_vs_oracle(gpu, oracle, source, needle, cs, mode):
+    key = random.Random(hash(needle) & 0xFFFF).randbytes(32)
+    first = 1 << 20
+    count = 1 << 20
+    say(f"search {needle} source={source} mode={mode}")
+    hits, tries = gpu.batch_run([_spec(key, source, needle, cs, mode, first, count)])
+    assert tries[0] >= count
+    want = oracle.search(key, source, first, tries[0], needle, cs, mode)
+    assert [h.index for h in hits] == want
+    pks = {h.index: h.pk for h in hits}
+    for i in want[:32]:
+        assert pks[i] == oracle.derive(key, source, i, 1)
+        assert [h.seed for h in hits if h.index == i][0] == oracle.seed(key, source, i)
+
+
+def test_sign_dependent_suffix(gpu, oracle):
+    """Suffixes of 2-3 chars cover tail position 42, whose value holds the x sign; the
+    exact hit set must follow the real sign."""
+    key = random.Random(43).randbytes(32)
+    for needle in (b"A", b"AB", b"gQ", b"Yw"):
+        hits, tries = gpu.batch_run([_spec(key, 0, needle, True, M.MODE_SUFFIX, 0, 1 << 19)])
+        want = oracle.search(key, 0, 0, tries[0], needle, True, 1)
+        assert [h.index for h in hits] == want, needle
+
+
+def test_engine_launch_ranges(gpu_small, oracle):
+    """Engine::launch semantics: a launch covers [base, base + keys_per_iter * iters)."""
+    g = gpu_small
+    key = random.Random(44).randbytes(32)
+    kpi = g.keys_per_iter
+    l = g.prepare(key, M.SEED_CHACHA20, b"xy", True, M.MODE_CONTAINS)
+    base = 3 * kpi + 17
+    got = [h.index for h in l.run(base, 2)]
+    want = oracle.search(key, 0, base, 2 * kpi, b"xy", False, 0)
+    assert got == want and len(want) > 0
+    # consecutive launches tile the index space
+    a = [h.index for h in l.run(0, 1)] + [h.index for h in l.run(kpi, 1)]
+    assert a == oracle.search(key, 0, 0, 2 * kpi, b"xy", False, 0)
+    l.close()
+
+
+def test_stop_on_hit_reports_a_true_hit(gpu_small, oracle):
+    g = gpu_small
+    key = random.Random(45).randbytes(32)
+    l = g.prepare(key, M.SEED_CHACHA20, b"q", True, M.MODE_CONTAINS, stop_on_hit=True)
+    hits = l.run(0, 4)
+    assert len(hits) >= 1
+    for h in hits:
+        assert oracle.derive(key, 0, h.index, 1) == h.pk
+        assert "q" in T.hay(h.pk).lower()  # fold = case-insensitive
+    l.close()
+
+
+def test_batched_equals_single(gpu, oracle):
+    """Searches batched in one launch return the same hit sets as one launch each."""
+    rng = random.Random(46)
+    specs = []
+    for i in range(8):
+        key = rng.randbytes(32)
+        needle = rng.choice([b"ab", b"Zz", b"9", b"k/"])
+        mode = rng.choice([M.MODE_CONTAINS, M.MODE_SUFFIX])
+        specs.append(_spec(key, i % 2, needle, i % 3 == 0, mode, rng.getrandbits(40), 64 * 32 * rng.randint(1, 40)))
+    batched, tries = gpu.batch_run(specs)
+    for i, s in enumerate(specs):
+        single, t1 = gpu.batch_run([s])
+        assert t1[0] == tries[i]
+        assert [h.index for h in batched if h.search == i] == [h.index for h in single]
+        want = oracle.search(s["key"], s["seed_source"], s["first"], tries[i], s["needle"], not s["fold"],
+                             s["mode"])
+        assert [h.index for h in single] == want
+
+
+def test_config1_full_hit_set(gpu, oracle):
+    """Config 1: 2^24 ChaCha20 candidates of the master key from PRNG seed 1, a seeded
+    4-char case-insensitive substring; every hit index and key equal to the oracle's."""
+    rng = random.Random(1)
+    key = rng.randbytes(32)
+    alpha = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
+    needle = "".join(rng.choice(alpha) for _ in range(4)).lower().encode()
+    n = 1 << 24
+    say(f"config1 needle={needle}")
+    hits, tries = gpu.batch_run([_spec(key, 0, needle, False, M.MODE_CONTAINS, 0, n)])
+    assert tries[0] == n
+    want = oracle.search(key, 0, 0, n, needle, False, 0, threads=16)
+    assert [h.index for h in hits] == want
+    for h in hits:
+        assert "".join(T.hay(h.pk)).lower().find(needle.decode()) >= 0
+    say(f"config1 hits={len(want)}")

@@ -1,0 +1,160 @@
+"""The kernel arithmetic (meikey_amd/csrc/mk_*.h) compiled for the host, against the
+oracle and Python integers: the same source the gfx950 kernels run, checked without a
+GPU, including the limb bounds the kernels rely on."""
+import random
+import struct
+
+import pytest
+
+from tests import textbook as T
+
+P = T.P
+MASK26, MASK25 = (1 << 26) - 1, (1 << 25) - 1
+
+
+def test_seed_and_scalar(oracle, hostcheck):
+    rng = random.Random(21)
+    for src in (0, 1):
+        key = rng.randbytes(32)
+        for i in [0, 1, 2, 3, 2**32 - 1, 2**32, 2**64 - 1] + [rng.getrandbits(64) for _ in range(20)]:
+            s = hostcheck.seed(key, src, i)
+            assert s == oracle.seed(key, src, i)
+            assert hostcheck.secret_scalar(s) == oracle.secret_scalar(s)
+
+
+def _rand_limbs(rng, even_max, odd_max):
+    return [rng.randrange(0, (even_max if i % 2 == 0 else odd_max) + 1) for i in range(10)]
+
+
+@pytest.mark.parametrize("case", ["carried", "sub", "t_side", "extreme"])
+def test_fe_mul_within_bounds(hostcheck, case):
+    """fe_mul is exact for the operand bounds the point formulas produce: g up to
+    3*2^26 / 3*2^25 per limb (a subtraction result), f up to 2^28 (the t operand)."""
+    rng = random.Random(hash(case) & 0xFFFF)
+    for _ in range(300):
+        if case == "carried":
+            f = _rand_limbs(rng, MASK26, MASK25)
+            g = _rand_limbs(rng, MASK26, MASK25)
+        elif case == "sub":
+            f = _rand_limbs(rng, 3 << 26, 3 << 25)
+            g = _rand_limbs(rng, 3 << 26, 3 << 25)
+        elif case == "t_side":
+            f = _rand_limbs(rng, 1 << 28, 1 << 27)
+            g = _rand_limbs(rng, (3 << 26) + (3 << 13), (3 << 25) + (3 << 13))
+        else:
+            f = [(1 << 28) if i % 2 == 0 else (1 << 27) for i in range(10)]
+            g = [(3 << 26) + (3 << 13) if i % 2 == 0 else (3 << 25) + (3 << 13) for i in range(10)]
+        r = hostcheck.fe_mul_limbs(f, g)
+        assert all(x < (1 << 26) + (1 << 14) for x in r)
+        assert hostcheck.fe_to_int(r) == hostcheck.value_of(f) * hostcheck.value_of(g) % P
+        q = hostcheck.fe_sq_limbs(r)
+        assert hostcheck.fe_to_int(q) == hostcheck.value_of(r) ** 2 % P
+
+
+def test_fe_ops_edge_values(oracle, hostcheck):
+    from tests.test_oracle import EDGE
+    rng = random.Random(22)
+    vals = EDGE + [rng.getrandbits(256) for _ in range(12)]
+    for a in vals:
+        for b in vals[:6]:
+            for op in (0, 1, 2):
+                assert hostcheck.fe_op(op, a, b) == oracle.fe_op(op, a, b), (op, a, b)
+        assert hostcheck.fe_op(3, a, 0) == a * a % P
+        assert hostcheck.fe_op(4, a, 0) == pow(a, P - 2, P)
+        assert hostcheck.fe_op(5, a, 0) == (-a) % P
+
+
+def test_to_words_canonical(hostcheck):
+    # values just below, at and above p in carried limb form
+    for v in [P - 1, P, P + 1, P + 18, 2**255 - 1, 0, 1]:
+        limbs = hostcheck.limbs_of(v % (1 << 255))
+        assert hostcheck.fe_to_int(limbs) == v % P
+
+
+def test_digits(oracle, hostcheck):
+    rng = random.Random(23)
+    for w in range(2, 25):
+        for _ in range(5):
+            s = oracle.secret_scalar(rng.randbytes(32))
+            assert hostcheck.digits(s, w) == oracle.digits(s, w), w
+
+
+def _entry_ints(words):
+    from tests.hostcheck import value_of
+    return [value_of(words[10 * f:10 * f + 10]) for f in range(3)]
+
+
+@pytest.mark.parametrize("w", [2, 3, 5, 8])
+def test_table_matches_oracle(oracle, hostcheck, w):
+    tab = hostcheck.table(w)
+    rows, cols = -(-251 // w), (1 << (w - 1)) + 1
+    rng = random.Random(w)
+    picks = [(0, 0), (0, 1), (rows - 1, 0), (rows - 1, cols - 1)] + [
+        (rng.randrange(rows), rng.randrange(cols)) for _ in range(12)]
+    for row, k in picks:
+        words = list(tab[(row * cols + k) * 32:(row * cols + k + 1) * 32])
+        ypx, ymx, third = _entry_ints(words)
+        n = oracle.comb_entry_niels(w, row, k)
+        assert ypx == int.from_bytes(n[0:32], "little")
+        assert ymx == int.from_bytes(n[32:64], "little")
+        xy2d = int.from_bytes(n[64:96], "little")
+        if row == 0:
+            assert third == xy2d * T.inv(T.D) % P  # row 0 keeps 2xy
+        else:
+            assert third == xy2d
+        assert words[30] == 0 and words[31] == 0
+        # canonical limbs
+        assert all(words[i] <= (MASK26 if (i % 10) % 2 == 0 else MASK25) for i in range(30))
+
+
+@pytest.mark.parametrize("w", [3, 7, 8])
+def test_full_pk_and_lane(oracle, hostcheck, w):
+    tab = hostcheck.table(w)
+    rng = random.Random(100 + w)
+    key = rng.randbytes(32)
+    for src in (0, 1):
+        first = rng.getrandbits(40)
+        K = 6
+        pks = oracle.derive(key, src, first, K)
+        for j in range(K):
+            s = oracle.secret_scalar(oracle.seed(key, src, first + j))
+            assert hostcheck.full_pk(tab, w, s) == pks[32 * j:32 * j + 32]
+        needle = T.hay(pks[0:32])[10:13].encode()
+        ys, cand = hostcheck.lane(tab, w, key, src, first, K, needle, False, 0)
+        for j in range(K):
+            pk = pks[32 * j:32 * j + 32]
+            assert ys[j] == pk[:31] + bytes([pk[31] & 0x7F])
+            assert cand[j] == int(oracle.match_unsigned(pk, needle, False, 0))
+        assert cand[0] == 1
+
+
+def test_bitap_matcher_vs_oracle(oracle, hostcheck):
+    rng = random.Random(24)
+    alpha = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
+    for _ in range(4000):
+        pk = rng.randbytes(32)
+        b64 = T.public_b64(pk)
+        L = rng.choice([1, 2, 3, 5, 8, 42, 43])
+        if rng.random() < 0.6:
+            st = rng.randrange(25, 69 - L)
+            needle = b64[st:st + L]
+        else:
+            needle = "".join(rng.choice(alpha) for _ in range(L))
+        if rng.random() < 0.3:
+            needle = needle.swapcase()
+        cs, mode = rng.random() < 0.5, rng.choice([0, 1])
+        nb = needle.encode()
+        assert hostcheck.match(pk, nb, not cs, mode, False) == oracle.match(b64, nb, cs, mode)
+        # sign-unknown matching takes y with the sign bit 0, as the search kernel has it
+        y = pk[:31] + bytes([pk[31] & 0x7F])
+        assert hostcheck.match(y, nb, not cs, mode, True) == oracle.match_unsigned(pk, nb, cs, mode)
+
+
+def test_bitap_mask_folds_case(hostcheck):
+    m = hostcheck.build_mask(b"aB", True)
+    alpha = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
+    assert m[alpha.index("a")] == 1 and m[alpha.index("A")] == 1
+    assert m[alpha.index("b")] == 2 and m[alpha.index("B")] == 2
+    assert sum(1 for x in m if x) == 4
+    m = hostcheck.build_mask(b"aB", False)
+    assert m[alpha.index("a")] == 1 and m[alpha.index("A")] == 0 and m[alpha.index("B")] == 2
