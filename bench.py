@@ -42,11 +42,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--width", type=int, default=16)
-    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--width", type=int, default=24)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--blocks", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_bytes:
+        copy_calibration(args.copy_bytes)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each search shards by index "
               "range across GPUs with no collectives; see DESIGN.md)", file=sys.stderr)
@@ -83,6 +88,7 @@ def main():
     avg_ms = sum(kernel_ms) / len(kernel_ms)
     mults = field_mults_per_key(args.width, args.batch)
     achieved = STEP_KEYS * mults * 64 / (avg_ms * 1e-3)
+    traffic = measured_traffic(args.width, args.batch, args.blocks)
     out = {
         "metric": METRIC, "value": rate, "unit": "keys/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
@@ -93,7 +99,7 @@ def main():
                    "keys_per_thread": args.batch, "waves_per_simd": args.blocks, "grid_waves": info.grid_waves,
                    "hits": len(all_hits)},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_MUL / 1e12, "unit": "TOP/s",
-                     "frac": achieved / PEAK_MUL, "traffic": None,
+                     "frac": achieved / PEAK_MUL, "traffic": traffic,
                      "kernel": "mk_search_kernel", "kernel_ms": avg_ms,
                      "algorithmic": f"{mults:.2f} field mults/key x 64 32x32-bit products"},
     }
@@ -101,6 +107,34 @@ def main():
         out["cpu_baseline"] = cpu_baseline(gpu, key, needle)
     gpu.close()
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(width: int, batch: int, blocks: int):
+    """HBM bytes per launch of the search kernel, from the committed FETCH_SIZE /
+    WRITE_SIZE counter runs (profiles/mk_search_kernel_traffic.json, each scaled by the
+    same counter's reading for a copy kernel of known size); None when those runs were
+    made for another setting."""
+    path = os.path.join(ROOT, "profiles", "mk_search_kernel_traffic.json")
+    try:
+        t = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if (t.get("width"), t.get("keys_per_thread"), t.get("waves_per_simd")) != (width, batch, blocks):
+        return None
+    return t["bytes_per_key"] * STEP_KEYS
+
+
+def copy_calibration(nbytes: int) -> None:
+    """A copy kernel of known size: read nbytes and write nbytes, three times. Its
+    FETCH_SIZE / WRITE_SIZE counters give the scale factor for the search kernel's."""
+    import torch
+    x = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    x.fill_(1)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "dispatches": 3}), flush=True)
 
 
 def cpu_baseline(gpu, key, needle):

@@ -57,7 +57,7 @@ typedef enum {
 /* Kernel settings: --gpu-width / --gpu-batch / --gpu-limbs (src/controller/cli.rs:85-111,
  * gpu.rs:185-240 GpuOptions). */
 typedef struct {
-    uint32_t width;  /* comb table width W, 2..24 (reference 2..20); table = rows(W) x (2^(W-1)+1) x 128 B */
+    uint32_t width;  /* comb table width W, 2..28 (reference 2..20); table = rows(W) x (2^(W-1)+1) x 128 B */
     uint32_t batch;  /* keys per thread per tile, 1..256 (one inversion per `batch` keys per lane) */
     uint32_t limbs;  /* MK_LIMBS_R25 */
     uint32_t blocks; /* resident waves per SIMD the grid is sized for, 1..8 (reference: blocks per SM) */
@@ -93,7 +93,8 @@ void mk_last_error(char* buf, size_t len);
 /* Number of HIP devices (device_count, gpu.rs:1127-1140). */
 mk_status mk_device_count(int* n);
 
-/* Default options (W = 16, batch 32, R25 limbs, 2 waves per SIMD). */
+/* Default options (W = 24, 64 keys per thread, R25 limbs, 2 waves per SIMD): the fastest
+ * setting measured on MI355X; the W = 24 table takes 11.8 GB of HBM. */
 void mk_default_options(mk_options* out);
 
 /* Open device `ordinal`: build the comb table of `opt->width` on the device and run the

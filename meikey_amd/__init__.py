@@ -13,7 +13,7 @@ from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmeikey_hip.so")
+LIB_PATH = os.environ.get("MEIKEY_HIP_LIB") or os.path.join(HERE, "libmeikey_hip.so")
 
 SEED_CHACHA20 = 0
 SEED_COUNTER = 1
@@ -170,7 +170,7 @@ class Launch:
 class Gpu:
     """One device with its comb table (the reference's Gpu + Kernel)."""
 
-    def __init__(self, ordinal: int = 0, width: int = 16, batch: int = 32, blocks: int = 2, limbs: int = LIMBS_R25):
+    def __init__(self, ordinal: int = 0, width: int = 24, batch: int = 64, blocks: int = 2, limbs: int = LIMBS_R25):
         self.options = Options(width, batch, limbs, blocks)
         h = ctypes.c_void_p()
         _check(lib().mk_open(ordinal, ctypes.byref(self.options), ctypes.byref(h)))

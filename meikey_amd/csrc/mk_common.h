@@ -42,7 +42,7 @@ typedef uint16_t u16;
 // 285-287 of the reference).
 #define MK_M_BITS 251
 #define MK_MIN_W 2
-#define MK_MAX_W 24
+#define MK_MAX_W 28
 #define MK_MAX_ROWS ((MK_M_BITS + MK_MIN_W - 1) / MK_MIN_W)
 
 MK_DEV u32 mk_rows(u32 w) { return (MK_M_BITS + w - 1) / w; }

@@ -56,7 +56,59 @@ MK_DEV void mk_counter_seed(const u32 key[8], u64 index, u32 seed[8]) {
 
 MK_DEV u32 mk_bswap32(u32 x) { return (x >> 24) | ((x >> 8) & 0xff00u) | ((x << 8) & 0xff0000u) | (x << 24); }
 
-#define MK_ROTR64(x, n) (((x) >> (n)) | ((x) << (64 - (n))))
+// 64-bit values as (hi, lo) 32-bit halves so that every rotate is two funnel shifts
+// (v_alignbit_b32) and three-input XOR / majority are one v_bitop3_b32 each on gfx950.
+struct mk_u64p {
+    u32 lo, hi;
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+MK_DEV u32 mk_funnel(u32 hi, u32 lo, u32 n) { return __builtin_amdgcn_alignbit(hi, lo, n); }
+MK_DEV u32 mk_xor3(u32 a, u32 b, u32 c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96); }
+MK_DEV u32 mk_maj(u32 a, u32 b, u32 c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0xE8); }
+#else
+// (hi:lo) >> n, low 32 bits, for n in 0..31
+MK_DEV u32 mk_funnel(u32 hi, u32 lo, u32 n) { return (u32)((((u64)hi << 32) | lo) >> (n & 31)); }
+MK_DEV u32 mk_xor3(u32 a, u32 b, u32 c) { return a ^ b ^ c; }
+MK_DEV u32 mk_maj(u32 a, u32 b, u32 c) { return (a & b) | (a & c) | (b & c); }
+#endif
+
+// rotate right by a constant n in 1..63
+MK_DEV mk_u64p mk_rotr(mk_u64p x, int n) {
+    mk_u64p r;
+    if (n < 32) {
+        r.lo = mk_funnel(x.hi, x.lo, (u32)n);
+        r.hi = mk_funnel(x.lo, x.hi, (u32)n);
+    } else {
+        r.lo = mk_funnel(x.lo, x.hi, (u32)(n - 32));
+        r.hi = mk_funnel(x.hi, x.lo, (u32)(n - 32));
+    }
+    return r;
+}
+
+// shift right by a constant n in 1..31
+MK_DEV mk_u64p mk_shr(mk_u64p x, int n) {
+    mk_u64p r;
+    r.lo = mk_funnel(x.hi, x.lo, (u32)n);
+    r.hi = x.hi >> n;
+    return r;
+}
+
+MK_DEV mk_u64p mk_x3(mk_u64p a, mk_u64p b, mk_u64p c) {
+    mk_u64p r;
+    r.lo = mk_xor3(a.lo, b.lo, c.lo);
+    r.hi = mk_xor3(a.hi, b.hi, c.hi);
+    return r;
+}
+
+MK_DEV u64 mk_v(mk_u64p x) { return (u64)x.hi << 32 | x.lo; }
+
+MK_DEV mk_u64p mk_p(u64 v) {
+    mk_u64p r;
+    r.lo = (u32)v;
+    r.hi = (u32)(v >> 32);
+    return r;
+}
 
 // FIPS 180-4 round constants.
 #define MK_SHA512_K                                                                                                 \
@@ -83,9 +135,50 @@ MK_DEV u32 mk_bswap32(u32 x) { return (x >> 24) | ((x >> 8) & 0xff00u) | ((x << 
 
 MK_CONST u64 MK_K512[80] = MK_SHA512_K;
 
+// One SHA-512 round t (compile-time), message schedule included. The eight working
+// variables rotate through the call sites instead of being moved, and w[] is indexed
+// only by constants, so the 80 rounds become straight-line code with no register
+// indexing.
+template <int T>
+MK_DEV void mk_sha_round(u64 w[16], u64 a, u64 b, u64 c, u64& d, u64 e, u64 f, u64 g, u64& h) {
+    u64 wt;
+    if (T < 16) {
+        wt = w[T];
+    } else {
+        const mk_u64p w15 = mk_p(w[(T + 1) & 15]), w2 = mk_p(w[(T + 14) & 15]);
+        const u64 s0 = mk_v(mk_x3(mk_rotr(w15, 1), mk_rotr(w15, 8), mk_shr(w15, 7)));
+        const u64 s1 = mk_v(mk_x3(mk_rotr(w2, 19), mk_rotr(w2, 61), mk_shr(w2, 6)));
+        wt = w[T & 15] + s0 + w[(T + 9) & 15] + s1;
+        w[T & 15] = wt;
+    }
+    const mk_u64p ep = mk_p(e), ap = mk_p(a), bp = mk_p(b), cp = mk_p(c);
+    const u64 S1 = mk_v(mk_x3(mk_rotr(ep, 14), mk_rotr(ep, 18), mk_rotr(ep, 41)));
+    const u64 ch = (e & f) ^ (~e & g);
+    const u64 t1 = h + S1 + ch + (MK_K512[T] + wt);
+    const u64 S0 = mk_v(mk_x3(mk_rotr(ap, 28), mk_rotr(ap, 34), mk_rotr(ap, 39)));
+    mk_u64p mj;
+    mj.lo = mk_maj(ap.lo, bp.lo, cp.lo);
+    mj.hi = mk_maj(ap.hi, bp.hi, cp.hi);
+    d += t1;                // the new e
+    h = t1 + S0 + mk_v(mj);  // the new a
+}
+
+// Eight rounds from T; afterwards the variables are back in their original roles.
+template <int T>
+MK_DEV void mk_sha_8rounds(u64 w[16], u64& a, u64& b, u64& c, u64& d, u64& e, u64& f, u64& g, u64& h) {
+    mk_sha_round<T + 0>(w, a, b, c, d, e, f, g, h);
+    mk_sha_round<T + 1>(w, h, a, b, c, d, e, f, g);
+    mk_sha_round<T + 2>(w, g, h, a, b, c, d, e, f);
+    mk_sha_round<T + 3>(w, f, g, h, a, b, c, d, e);
+    mk_sha_round<T + 4>(w, e, f, g, h, a, b, c, d);
+    mk_sha_round<T + 5>(w, d, e, f, g, h, a, b, c);
+    mk_sha_round<T + 6>(w, c, d, e, f, g, h, a, b);
+    mk_sha_round<T + 7>(w, b, c, d, e, f, g, h, a);
+}
+
 // clamp(SHA-512(seed)[..32]) for a 32-byte seed given as 8 LE words; the scalar is
 // returned as 8 LE words. The message is one block: the 4 seed words, 0x80 padding
-// and the bit length 256, so w[5..14] are zero and fold away in the unrolled rounds.
+// and the bit length 256, so w[5..14] are zero and fold away.
 MK_DEV void mk_secret_scalar(const u32 seed[8], u32 s_out[8]) {
     const u64 iv[8] = MK_SHA512_IV;
     u64 w[16];
@@ -96,26 +189,16 @@ MK_DEV void mk_secret_scalar(const u32 seed[8], u32 s_out[8]) {
     for (int i = 5; i < 15; i++) w[i] = 0;
     w[15] = 256;
     u64 a = iv[0], b = iv[1], c = iv[2], d = iv[3], e = iv[4], f = iv[5], g = iv[6], h = iv[7];
-    MK_UNROLL
-    for (int t = 0; t < 80; t++) {
-        u64 wt;
-        if (t < 16) {
-            wt = w[t];
-        } else {
-            const u64 w15 = w[(t + 1) & 15], w2 = w[(t + 14) & 15];
-            const u64 s0 = MK_ROTR64(w15, 1) ^ MK_ROTR64(w15, 8) ^ (w15 >> 7);
-            const u64 s1 = MK_ROTR64(w2, 19) ^ MK_ROTR64(w2, 61) ^ (w2 >> 6);
-            wt = w[t & 15] + s0 + w[(t + 9) & 15] + s1;
-            w[t & 15] = wt;
-        }
-        const u64 S1 = MK_ROTR64(e, 14) ^ MK_ROTR64(e, 18) ^ MK_ROTR64(e, 41);
-        const u64 ch = (e & f) ^ (~e & g);
-        const u64 t1 = h + S1 + ch + MK_K512[t] + wt;
-        const u64 S0 = MK_ROTR64(a, 28) ^ MK_ROTR64(a, 34) ^ MK_ROTR64(a, 39);
-        const u64 mj = (a & b) ^ (a & c) ^ (b & c);
-        const u64 t2 = S0 + mj;
-        h = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + t2;
-    }
+    mk_sha_8rounds<0>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<8>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<16>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<24>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<32>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<40>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<48>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<56>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<64>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<72>(w, a, b, c, d, e, f, g, h);
     const u64 H[4] = {a + iv[0], b + iv[1], c + iv[2], d + iv[3]};
     MK_UNROLL
     for (int i = 0; i < 4; i++) {
@@ -126,15 +209,25 @@ MK_DEV void mk_secret_scalar(const u32 seed[8], u32 s_out[8]) {
     s_out[7] = (s_out[7] & 0x7fffffffu) | 0x40000000u;
 }
 
-// Seed of index i for source 0 (ChaCha20) or 1 (counter); `block` caches the ChaCha20
-// block of the even index so the odd index that follows reuses it.
-MK_DEV void mk_seed_at(const u32 key[8], u32 source, u64 index, u32 block[16], bool fresh, u32 seed[8]) {
+// Seed of index i for source 0 (ChaCha20) or 1 (counter). `upper` keeps the second
+// half of the last ChaCha20 block computed, which is the seed of the odd index that
+// follows an even one; `fresh` forces a new block.
+MK_DEV void mk_seed_at(const u32 key[8], u32 source, u64 index, u32 upper[8], bool fresh, u32 seed[8]) {
     if (source == 1) {
         mk_counter_seed(key, index, seed);
         return;
     }
-    if (fresh || (index & 1) == 0) mk_chacha20_block(key, index >> 1, block);
-    const bool upper = (index & 1) != 0;
+    if (fresh || (index & 1) == 0) {
+        u32 block[16];
+        mk_chacha20_block(key, index >> 1, block);
+        const bool odd = (index & 1) != 0;
+        MK_UNROLL
+        for (int i = 0; i < 8; i++) {
+            seed[i] = odd ? block[8 + i] : block[i];
+            upper[i] = block[8 + i];
+        }
+        return;
+    }
     MK_UNROLL
-    for (int i = 0; i < 8; i++) seed[i] = upper ? block[8 + i] : block[i];
+    for (int i = 0; i < 8; i++) seed[i] = upper[i];
 }

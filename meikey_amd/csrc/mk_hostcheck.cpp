@@ -22,8 +22,8 @@
 extern "C" {
 
 void hc_seed(const u32 key[8], u32 source, u64 index, u32 seed[8]) {
-    u32 block[16];
-    mk_seed_at(key, source, index, block, true, seed);
+    u32 upper[8];
+    mk_seed_at(key, source, index, upper, true, seed);
 }
 
 void hc_secret_scalar(const u32 seed[8], u32 s[8]) { mk_secret_scalar(seed, s); }
@@ -109,7 +109,11 @@ int hc_lane(const u32* table, u32 w, const u32 key[8], u32 source, u64 first, u3
     mk_spill sp;
     sp.base = spill.data();
     sp.stride = 1;
-    mk_lane_tile(table, w, K, key, source, first, sp, [&](u32 j, const u32 pk[8]) {
+    auto mul = [&](const u32 s[8], fe& yn, fe& yd) {
+        fe xn, xd;
+        mk_mul_base_frac(table, w, s, yn, yd, xn, xd);
+    };
+    mk_lane_tile(K, key, source, first, sp, mul, [&](u32 j, const u32 pk[8]) {
         memcpy(ys + 8 * j, pk, 32);
         cand[j] = mk_pk_match(pk, mt, true) ? 1 : 0;
     });

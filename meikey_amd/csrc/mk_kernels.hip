@@ -56,9 +56,9 @@ extern "C" __global__ void __launch_bounds__(64) mk_derive_kernel(const u32* __r
                                                                   u32* __restrict__ out_seed) {
     const u64 i = (u64)blockIdx.x * 64 + threadIdx.x;
     if (i >= n) return;
-    u32 k[8], block[16], seed[8], s[8], pk[8];
+    u32 k[8], upper[8], seed[8], s[8], pk[8];
     for (int j = 0; j < 8; j++) k[j] = key[j];
-    mk_seed_at(k, source, first + i, block, true, seed);
+    mk_seed_at(k, source, first + i, upper, true, seed);
     mk_secret_scalar(seed, s);
     mk_full_pk(table, w, s, pk);
     for (int j = 0; j < 8; j++) out_pk[i * 8 + j] = pk[j];
@@ -87,12 +87,19 @@ __device__ static u64 mk_uniform64(u64 v) {
     return (u64)hi << 32 | lo;
 }
 
-extern "C" __global__ void __launch_bounds__(64)
+// Resident waves per SIMD the search kernel's register budget is sized for
+// (512 VGPRs per SIMD lane are shared by the waves: 2 waves -> 256 each).
+#ifndef MK_SEARCH_WAVES
+#define MK_SEARCH_WAVES 2
+#endif
+
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
     mk_search_kernel(const u32* __restrict__ table, u32 w, u32 K, const mk_search_desc* __restrict__ descs,
                      const mk_seg* __restrict__ segs, u32 nsegs, u64 total_tiles, const u64* __restrict__ masks,
                      const u16* __restrict__ dfas, u32* __restrict__ scratch, mk_counters* __restrict__ ctr,
                      mk_cand* __restrict__ cands, u32 cand_cap, u32* __restrict__ stop) {
     __shared__ u64 lmask[64];
+    __shared__ uint4 stage[2 * 8 * 64];
     const u32 lane = threadIdx.x;
     const u64 lanes = (u64)gridDim.x * 64;
     const u64 gl = (u64)blockIdx.x * 64 + lane;
@@ -131,7 +138,11 @@ extern "C" __global__ void __launch_bounds__(64)
         mk_spill sp;
         sp.base = scratch + gl;
         sp.stride = lanes;
-        mk_lane_tile(table, w, K, key, dsc.source, first, sp, [&](u32 jj, const u32 pk[8]) {
+        auto mul = [&](const u32 s[8], fe& yn, fe& yd) {
+            fe xn, xd;
+            mk_mul_base_frac_staged(table, w, s, stage, lane, yn, yd, xn, xd);
+        };
+        mk_lane_tile(K, key, dsc.source, first, sp, mul, [&](u32 jj, const u32 pk[8]) {
             const bool cand = mk_pk_match(pk, mt, true);
             const u64 bal = __ballot(cand);
             if (bal == 0) return;
@@ -168,9 +179,9 @@ extern "C" __global__ void __launch_bounds__(64)
     if (i >= n) return;
     const mk_cand c = cands[i];
     const mk_search_desc& dsc = descs[c.search];
-    u32 key[8], block[16], seed[8], s[8], pk[8];
+    u32 key[8], upper[8], seed[8], s[8], pk[8];
     for (int j = 0; j < 8; j++) key[j] = dsc.key[j];
-    mk_seed_at(key, dsc.source, c.index, block, true, seed);
+    mk_seed_at(key, dsc.source, c.index, upper, true, seed);
     mk_secret_scalar(seed, s);
     mk_full_pk(table, w, s, pk);
     mk_matcher mt;

@@ -30,10 +30,12 @@ MK_DEV mk_ext mk_add_niels(const mk_ext& p, const mk_niels& q) {
     const fe x = fe_sub(pp, mm), y = fe_add(pp, mm);
     const fe z = fe_add(z2, txy), t = fe_sub(z2, txy);
     mk_ext r;
-    r.x = fe_mul(t, x);  // t (limbs < 2^28) only ever on the first operand
+    // t (limbs < 2^28) only ever on the first operand; x and z are each the second
+    // operand of two products, so their 19x multiples are computed once
+    r.x = fe_mul(t, x);
     r.y = fe_mul(y, z);
     r.z = fe_mul(t, z);
-    r.t = fe_mul(x, y);
+    r.t = fe_mul(y, x);
     return r;
 }
 
@@ -174,6 +176,75 @@ MK_DEV void mk_mul_base_frac(const u32* __restrict__ table, u32 w, const u32 s[8
     }
     mk_add_niels_frac(acc, mk_entry_niels(next, dn), yn, yd, xn, xd);
 }
+
+#if defined(__HIPCC__)
+// Wait for every outstanding memory operation of this wave (the LDS copies included).
+MK_DEV void mk_wait_all() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_s_waitcnt(0);
+#endif
+}
+
+// LDS staging of the table gathers (gfx950 global_load_lds_dwordx4): the next row's
+// entry is copied straight from HBM into LDS without passing through VGPRs, so the
+// prefetch costs no registers. `stage` is this wave's 2 x 8 KB double buffer laid out
+// [buffer][16-byte chunk][lane]; chunk q of lane l sits at stage[(b * 8 + q) * 64 + l],
+// so both the copy (one instruction per chunk for the whole wave) and the read
+// (ds_read_b128, consecutive lanes on consecutive 16-byte words) are conflict-free.
+MK_DEV void mk_stage_entry(const u32* __restrict__ table, u64 row, i32 d, u32 cols, uint4* stage, u32 b) {
+    const u32 k = (u32)(d < 0 ? -d : d);
+    const u32* e = table + (row * cols + k) * MK_ENTRY_WORDS;
+#if defined(__HIP_DEVICE_COMPILE__)
+    MK_UNROLL
+    for (int q = 0; q < MK_ENTRY_WORDS / 4; q++)
+        __builtin_amdgcn_global_load_lds((const void*)(e + 4 * q), (void*)(stage + (b * 8 + q) * 64), 16, 0, 0);
+#else
+    (void)e;
+    (void)stage;
+    (void)b;
+#endif
+}
+
+MK_DEV mk_niels mk_staged_niels(const uint4* stage, u32 b, u32 lane, i32 d) {
+    mk_entry e;
+    MK_UNROLL
+    for (int q = 0; q < MK_ENTRY_WORDS / 4; q++) {
+        const uint4 v = stage[(b * 8 + q) * 64 + lane];
+        e.w[4 * q] = v.x;
+        e.w[4 * q + 1] = v.y;
+        e.w[4 * q + 2] = v.z;
+        e.w[4 * q + 3] = v.w;
+    }
+    return mk_entry_niels(e, d);
+}
+
+// mk_mul_base_frac with the gathers staged through LDS: the entry of row r + 1 is in
+// flight while row r is added. Every wave of the kernel calls this with all 64 lanes.
+MK_DEV void mk_mul_base_frac_staged(const u32* __restrict__ table, u32 w, const u32 s[8], uint4* stage, u32 lane,
+                                    fe& yn, fe& yd, fe& xn, fe& xd) {
+    mk_digits g = mk_digits_of(s, w);
+    const u32 rows = g.rows, cols = mk_cols(w);
+    i32 d = mk_next_digit(g);
+    mk_stage_entry(table, 0, d, cols, stage, 0);
+    i32 dn = mk_next_digit(g);
+    mk_stage_entry(table, 1, dn, cols, stage, 1);
+    mk_wait_all();
+    mk_ext acc = mk_row0_extended(mk_staged_niels(stage, 0, lane, d));
+    MK_NO_UNROLL
+    for (u32 row = 1; row + 1 < rows; row++) {
+        // the copy of this row's entry (issued one row ago) must have landed
+        mk_wait_all();
+        const mk_niels cur = mk_staged_niels(stage, row & 1, lane, dn);
+        d = dn;
+        dn = mk_next_digit(g);
+        // the other buffer was read into registers during the previous row
+        mk_stage_entry(table, row + 1, dn, cols, stage, (row + 1) & 1);
+        acc = mk_add_niels(acc, cur);
+    }
+    mk_wait_all();
+    mk_add_niels_frac(acc, mk_staged_niels(stage, (rows - 1) & 1, lane, dn), yn, yd, xn, xd);
+}
+#endif
 
 // Full compressed public key (y with the x sign) of one scalar; one inversion.
 MK_DEV void mk_full_pk(const u32* __restrict__ table, u32 w, const u32 s[8], u32 pk[8]) {

@@ -27,21 +27,20 @@ struct mk_spill {
     }
 };
 
-// Keys first .. first + K - 1: seed -> scalar -> (yn, yd); one inversion of the
-// product of the yd; then, from the last key to the first, y = yn / yd is compressed
-// (x sign bit left 0) and on_key(j, pk_words) is called.
-template <class F>
-MK_DEV void mk_lane_tile(const u32* __restrict__ table, u32 w, u32 K, const u32 key[8], u32 source, u64 first,
-                         const mk_spill& sp, F on_key) {
-    u32 block[16];
+// Keys first .. first + K - 1: seed -> scalar -> (yn, yd) by mul(s, yn, yd); one
+// inversion of the product of the yd; then, from the last key to the first, y = yn / yd
+// is compressed (x sign bit left 0) and on_key(j, pk_words) is called.
+template <class M, class F>
+MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const mk_spill& sp, M mul, F on_key) {
+    u32 upper[8];
     fe pre;
     MK_NO_UNROLL
     for (u32 j = 0; j < K; j++) {
         u32 seed[8], s[8];
-        mk_seed_at(key, source, first + j, block, j == 0, seed);
+        mk_seed_at(key, source, first + j, upper, j == 0, seed);
         mk_secret_scalar(seed, s);
-        fe yn, yd, xn, xd;
-        mk_mul_base_frac(table, w, s, yn, yd, xn, xd);
+        fe yn, yd;
+        mul(s, yn, yd);
         pre = j == 0 ? yd : fe_mul(pre, yd);
         sp.store(3 * j, yn);
         sp.store(3 * j + 1, yd);
