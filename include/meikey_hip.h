@@ -165,6 +165,82 @@ uint64_t mk_tile_keys(const mk_gpu* g);
  * events recorded on the library's stream around each kernel. */
 mk_status mk_last_timing(const mk_gpu* g, float* search_ms, float* confirm_ms);
 
+/* ------------------------------------------------------------------------------------
+ * Serving: continuous batching of search requests (no reference counterpart). Requests
+ * are admitted into device slots as slots and DFA-pool memory free up; each step is one
+ * launch over a fixed number of tiles dealt round-robin across the active requests, whose
+ * next key index and stop flag live on the device; the step is replayed from a captured
+ * hipGraph. A request ends at its first exact hit (its lowest index found in that step),
+ * when `count` candidates are spent (count 0 = unbounded), or when cancelled. The server
+ * uses the mk_gpu's stream and buffers: make no other call on that mk_gpu while it is open. */
+typedef struct {
+    uint32_t max_slots;  /* requests on the device at once, 1..4096 */
+    uint32_t use_graph;  /* 1: replay the captured step graph; 0: enqueue each step */
+    uint64_t dfa_budget; /* HBM bytes for the admitted requests' DFA tables */
+    uint64_t step_keys;  /* candidates per step (rounded up to whole tiles, at least one per wave) */
+} mk_server_options;
+
+typedef struct {
+    uint64_t request;
+    int32_t status; /* 1 found, 0 candidate budget spent, -1 invalid pattern or cancelled */
+    uint32_t pad;
+    uint64_t tries; /* candidates searched for this request */
+    double submit_s, admit_s, done_s; /* seconds since mk_server_open */
+    mk_hit_rec hit; /* when status == 1 */
+} mk_server_result;
+
+typedef struct mk_server mk_server;
+
+void mk_server_default_options(mk_server_options* out);
+mk_status mk_server_open(mk_gpu* g, const mk_server_options* opt, mk_server** out);
+/* Queue a request (stop_on_hit is implied); *request receives its id. */
+mk_status mk_server_submit(mk_server* s, const mk_search_spec* spec, uint64_t* request);
+/* Admit what fits, run one step, retire finished requests. */
+mk_status mk_server_step(mk_server* s, uint32_t* active, uint32_t* queued);
+/* Take up to cap finished results. */
+mk_status mk_server_poll(mk_server* s, mk_server_result* out, uint32_t cap, uint32_t* n);
+mk_status mk_server_cancel(mk_server* s, uint64_t request);
+mk_status mk_server_stats(const mk_server* s, uint64_t* steps, uint64_t* step_keys);
+double mk_server_now(const mk_server* s);
+mk_status mk_server_close(mk_server* s);
+
+/* ------------------------------------------------------------------------------------
+ * Patterns: Pattern::with_mode / probability / is_match and Dfa::new / gpu_table
+ * (src/model/pattern.rs:115-262, src/model/pattern/dfa.rs:52-200), host-side. The
+ * needle a pattern yields is what mk_prepare / mk_search_spec take (Needle::new,
+ * gpu.rs:627-659): the lower-cased text when case-insensitive, or the DFA table. */
+
+/* PatternError kinds (src/model/pattern.rs:23-55). */
+typedef enum {
+    MK_PATTERN_OK = 0,
+    MK_PATTERN_EMPTY = 1,
+    MK_PATTERN_INVALID_CHAR = 2,
+    MK_PATTERN_TOO_LONG = 3,
+    MK_PATTERN_REGEX = 4,
+    MK_PATTERN_TOO_COMPLEX = 5,
+    MK_PATTERN_MATCHES_EMPTY = 6,
+    MK_PATTERN_NEVER_MATCHES = 7
+} mk_pattern_error;
+
+typedef struct mk_pattern mk_pattern;
+
+/* Validate and compile `text` (text_len bytes) in mode MK_MODE_CONTAINS / SUFFIX / REGEX.
+ * On failure returns MK_ERR_INVALID, sets *error_kind and the text of
+ * mk_pattern_last_error(). */
+mk_status mk_pattern_new(const char* text, uint32_t text_len, int case_sensitive, uint32_t mode, mk_pattern** out,
+                         int* error_kind);
+void mk_pattern_last_error(char* buf, size_t len);
+mk_status mk_pattern_free(mk_pattern* p);
+/* The needle for mk_prepare: bytes (owned by the pattern), length (characters, or DFA
+ * states for a regex), fold and mode. mk_pattern_needle_bytes gives the byte size. */
+mk_status mk_pattern_needle(const mk_pattern* p, const uint8_t** bytes, uint32_t* len, uint32_t* fold, uint32_t* mode);
+size_t mk_pattern_needle_bytes(const mk_pattern* p);
+/* Chance that one key matches (Pattern::probability); 1 / this is the expected tries. */
+double mk_pattern_probability(const mk_pattern* p);
+uint32_t mk_pattern_dfa_states(const mk_pattern* p);
+/* Pattern::is_match on the 68-character public-key base64 (1 / 0, -1 on bad input). */
+int mk_pattern_is_match(const mk_pattern* p, const char* b64);
+
 #ifdef __cplusplus
 }
 #endif

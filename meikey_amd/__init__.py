@@ -38,7 +38,14 @@ EXPORTS = (
     "mk_last_error", "mk_device_count", "mk_default_options", "mk_open", "mk_info_get", "mk_close",
     "mk_keys_per_iter", "mk_prepare", "mk_launch_run", "mk_free_launch", "mk_derive", "mk_derive_seeds",
     "mk_table_read", "mk_batch_run", "mk_tile_keys", "mk_last_timing",
+    "mk_pattern_new", "mk_pattern_last_error", "mk_pattern_free", "mk_pattern_needle", "mk_pattern_needle_bytes",
+    "mk_pattern_probability", "mk_pattern_dfa_states", "mk_pattern_is_match",
+    "mk_server_default_options", "mk_server_open", "mk_server_submit", "mk_server_step", "mk_server_poll",
+    "mk_server_cancel", "mk_server_stats", "mk_server_now", "mk_server_close",
 )
+
+PATTERN_ERRORS = {1: "Empty", 2: "InvalidChar", 3: "TooLong", 4: "Regex", 5: "TooComplex", 6: "MatchesEmpty",
+                  7: "NeverMatches"}
 
 
 class MeikeyError(RuntimeError):
@@ -69,6 +76,17 @@ class SearchSpec(ctypes.Structure):
                 ("fold", ctypes.c_uint32), ("needle_len", ctypes.c_uint32), ("needle", ctypes.c_char_p),
                 ("stop_on_hit", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("first", ctypes.c_uint64),
                 ("count", ctypes.c_uint64)]
+
+
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("max_slots", ctypes.c_uint32), ("use_graph", ctypes.c_uint32), ("dfa_budget", ctypes.c_uint64),
+                ("step_keys", ctypes.c_uint64)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("request", ctypes.c_uint64), ("status", ctypes.c_int32), ("pad", ctypes.c_uint32),
+                ("tries", ctypes.c_uint64), ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("hit", HitRec)]
 
 
 @dataclass(frozen=True)
@@ -110,12 +128,32 @@ def lib() -> ctypes.CDLL:
     L.mk_derive_seeds.argtypes = [P, ctypes.c_char_p, U32, P]
     L.mk_table_read.argtypes = [P, U32, U32, U32, P]
     L.mk_last_timing.argtypes = [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    L.mk_pattern_new.argtypes = [ctypes.c_char_p, U32, I32, U32, ctypes.POINTER(P), ctypes.POINTER(I32)]
+    L.mk_pattern_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.mk_pattern_last_error.restype = None
+    L.mk_pattern_free.argtypes = [P]
+    L.mk_pattern_needle.argtypes = [P, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)), ctypes.POINTER(U32),
+                                    ctypes.POINTER(U32), ctypes.POINTER(U32)]
+    L.mk_pattern_needle_bytes.argtypes = [P]
+    L.mk_pattern_needle_bytes.restype = ctypes.c_size_t
+    L.mk_pattern_probability.argtypes = [P]
+    L.mk_pattern_probability.restype = ctypes.c_double
+    L.mk_pattern_dfa_states.argtypes = [P]
+    L.mk_pattern_dfa_states.restype = U32
+    L.mk_pattern_is_match.argtypes = [P, ctypes.c_char_p]
+    L.mk_server_default_options.argtypes = [ctypes.POINTER(ServerOptions)]
+    L.mk_server_default_options.restype = None
+    L.mk_server_open.argtypes = [P, ctypes.POINTER(ServerOptions), ctypes.POINTER(P)]
+    L.mk_server_submit.argtypes = [P, ctypes.POINTER(SearchSpec), ctypes.POINTER(U64)]
+    L.mk_server_step.argtypes = [P, ctypes.POINTER(U32), ctypes.POINTER(U32)]
+    L.mk_server_poll.argtypes = [P, ctypes.POINTER(ServerResult), U32, ctypes.POINTER(U32)]
+    L.mk_server_cancel.argtypes = [P, U64]
+    L.mk_server_stats.argtypes = [P, ctypes.POINTER(U64), ctypes.POINTER(U64)]
+    L.mk_server_now.argtypes = [P]
+    L.mk_server_now.restype = ctypes.c_double
+    L.mk_server_close.argtypes = [P]
     L.mk_batch_run.argtypes = [P, ctypes.POINTER(SearchSpec), U32, ctypes.POINTER(HitRec), U32,
                                ctypes.POINTER(U32), ctypes.POINTER(U64)]
-    for name in EXPORTS:
-        fn = getattr(L, name)
-        if fn.restype is ctypes.c_int and name not in ("mk_keys_per_iter", "mk_tile_keys"):
-            fn.restype = I32
     _lib = L
     return L
 
@@ -137,8 +175,86 @@ def device_count() -> int:
     return n.value
 
 
+def _fill_spec(dst: "SearchSpec", s: dict, keep: list) -> None:
+    needle = s["needle"]
+    keep.append(needle)
+    dst.key[:] = list(s["key"])
+    dst.seed_source = s.get("seed_source", SEED_CHACHA20)
+    dst.mode = s.get("mode", MODE_CONTAINS)
+    dst.fold = int(s.get("fold", False))
+    dst.needle_len = s.get("needle_len", len(needle) // 130 if dst.mode == MODE_REGEX else len(needle))
+    dst.needle = needle
+    dst.stop_on_hit = int(s.get("stop_on_hit", False))
+    dst.first = s.get("first", 0)
+    dst.count = s.get("count", 0)
+
+
 def _hits(recs, n: int) -> List[Hit]:
     return [Hit(recs[i].index, recs[i].search, bytes(recs[i].seed), bytes(recs[i].pk)) for i in range(n)]
+
+
+class PatternError(ValueError):
+    def __init__(self, kind: int, message: str):
+        super().__init__(f"{PATTERN_ERRORS.get(kind, kind)}: {message}")
+        self.kind = PATTERN_ERRORS.get(kind, str(kind))
+        self.message = message
+
+
+class Pattern:
+    """A validated search pattern (the reference's Pattern::with_mode): substring, suffix
+    or regular expression, case-sensitive or not. `needle()` gives what the kernels take."""
+
+    def __init__(self, text: str, case_sensitive: bool = False, mode: int = MODE_CONTAINS):
+        raw = text.encode("utf-8")
+        h = ctypes.c_void_p()
+        kind = ctypes.c_int(0)
+        st = lib().mk_pattern_new(raw, len(raw), int(case_sensitive), mode, ctypes.byref(h), ctypes.byref(kind))
+        if st != 0:
+            buf = ctypes.create_string_buffer(1024)
+            lib().mk_pattern_last_error(buf, len(buf))
+            raise PatternError(kind.value, buf.value.decode(errors="replace"))
+        self._h = h
+        self.text, self.case_sensitive, self.mode = text, case_sensitive, mode
+
+    def needle(self):
+        """(needle bytes, needle_len, fold, mode) for Gpu.prepare / Gpu.batch_run."""
+        ptr = ctypes.POINTER(ctypes.c_uint8)()
+        n, fold, mode = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().mk_pattern_needle(self._h, ctypes.byref(ptr), ctypes.byref(n), ctypes.byref(fold),
+                                       ctypes.byref(mode)))
+        size = lib().mk_pattern_needle_bytes(self._h)
+        return bytes(ptr[:size]), n.value, bool(fold.value), mode.value
+
+    def spec(self, key: bytes, first: int, count: int, seed_source: int = SEED_CHACHA20, stop_on_hit=False) -> dict:
+        needle, n, fold, mode = self.needle()
+        return {"key": key, "seed_source": seed_source, "needle": needle, "needle_len": n, "fold": fold,
+                "mode": mode, "first": first, "count": count, "stop_on_hit": stop_on_hit}
+
+    @property
+    def probability(self) -> float:
+        return lib().mk_pattern_probability(self._h)
+
+    @property
+    def expected_tries(self) -> float:
+        return 1.0 / self.probability
+
+    @property
+    def dfa_states(self) -> int:
+        return lib().mk_pattern_dfa_states(self._h)
+
+    def is_match(self, b64: str) -> bool:
+        r = lib().mk_pattern_is_match(self._h, b64.encode())
+        if r < 0:
+            raise ValueError("bad base64 text")
+        return bool(r)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().mk_pattern_free(self._h)
+                self._h = None
+        except Exception:
+            pass
 
 
 class Launch:
@@ -148,7 +264,7 @@ class Launch:
         self.gpu = gpu
         self._h = handle
 
-    def run(self, base: int, iters: int, cap: int = 1 << 16) -> List[Hit]:
+    def run(self, base: int, iters: int, cap: int = 1 << 20) -> List[Hit]:
         """Engine::launch: exact hits of [base, base + keys_per_iter * iters), ascending."""
         recs = (HitRec * max(cap, 1))()
         n = ctypes.c_uint32(0)
@@ -224,24 +340,14 @@ class Gpu:
         _check(lib().mk_table_read(self._h, row, k, count, buf))
         return list(buf)
 
-    def batch_run(self, specs: Sequence[dict], cap: int = 1 << 16):
+    def batch_run(self, specs: Sequence[dict], cap: int = 1 << 20):
         """Run many searches in one launch. Each spec: key, seed_source, mode, fold, needle,
         first, count, stop_on_hit (optional). Returns (hits, tries per search)."""
         n = len(specs)
         arr = (SearchSpec * max(n, 1))()
         keep = []
         for i, s in enumerate(specs):
-            needle = s["needle"]
-            keep.append(needle)
-            arr[i].key[:] = list(s["key"])
-            arr[i].seed_source = s.get("seed_source", SEED_CHACHA20)
-            arr[i].mode = s.get("mode", MODE_CONTAINS)
-            arr[i].fold = int(s.get("fold", False))
-            arr[i].needle_len = s.get("needle_len", len(needle) // 130 if arr[i].mode == MODE_REGEX else len(needle))
-            arr[i].needle = needle
-            arr[i].stop_on_hit = int(s.get("stop_on_hit", False))
-            arr[i].first = s.get("first", 0)
-            arr[i].count = s["count"]
+            _fill_spec(arr[i], s, keep)
         recs = (HitRec * max(cap, 1))()
         nh = ctypes.c_uint32(0)
         tries = (ctypes.c_uint64 * max(n, 1))()
@@ -251,6 +357,80 @@ class Gpu:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().mk_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass(frozen=True)
+class ServedResult:
+    request: int
+    status: int  # 1 found, 0 budget spent, -1 invalid or cancelled
+    tries: int
+    submit_s: float
+    admit_s: float
+    done_s: float
+    index: int
+    seed: bytes
+    pk: bytes
+
+
+class Server:
+    """Continuous batching of search requests on one Gpu (mk_server_* of the C ABI)."""
+
+    def __init__(self, gpu: Gpu, max_slots: int = 1024, use_graph: bool = True, dfa_budget: int = 64 << 20,
+                 step_keys: int = 1 << 26):
+        self.gpu = gpu
+        opt = ServerOptions(max_slots, int(use_graph), dfa_budget, step_keys)
+        h = ctypes.c_void_p()
+        _check(lib().mk_server_open(gpu._h, ctypes.byref(opt), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, spec: dict) -> int:
+        arr = SearchSpec()
+        keep = []
+        _fill_spec(arr, spec, keep)
+        rid = ctypes.c_uint64(0)
+        _check(lib().mk_server_submit(self._h, ctypes.byref(arr), ctypes.byref(rid)))
+        return rid.value
+
+    def step(self):
+        """One device step; returns (active, queued) before the step."""
+        a, q = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        _check(lib().mk_server_step(self._h, ctypes.byref(a), ctypes.byref(q)))
+        return a.value, q.value
+
+    def poll(self, cap: int = 4096) -> List[ServedResult]:
+        out = (ServerResult * cap)()
+        n = ctypes.c_uint32(0)
+        _check(lib().mk_server_poll(self._h, out, cap, ctypes.byref(n)))
+        return [ServedResult(r.request, r.status, r.tries, r.submit_s, r.admit_s, r.done_s, r.hit.index,
+                             bytes(r.hit.seed), bytes(r.hit.pk)) for r in out[:n.value]]
+
+    def cancel(self, request: int) -> None:
+        _check(lib().mk_server_cancel(self._h, request))
+
+    def now(self) -> float:
+        return lib().mk_server_now(self._h)
+
+    def stats(self):
+        steps, keys = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(lib().mk_server_stats(self._h, ctypes.byref(steps), ctypes.byref(keys)))
+        return steps.value, keys.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mk_server_close(self._h)
             self._h = None
 
     def __enter__(self):

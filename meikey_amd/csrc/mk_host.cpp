@@ -10,14 +10,15 @@
 #include <string>
 #include <vector>
 
-#include "../../include/meikey_hip.h"
-#include "mk_launch.h"
+#include "mk_internal.h"
 
 namespace {
 
 thread_local std::string g_last_error;
 
-mk_status fail(mk_status st, const char* fmt, ...) {
+}  // namespace
+
+mk_status mk_fail(mk_status st, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -26,6 +27,10 @@ mk_status fail(mk_status st, const char* fmt, ...) {
     g_last_error = buf;
     return st;
 }
+
+namespace {
+
+#define fail mk_fail
 
 #define MK_HIP(call)                                                                                 \
     do {                                                                                             \
@@ -41,74 +46,22 @@ mk_status fail(mk_status st, const char* fmt, ...) {
 
 namespace {
 
-constexpr u32 kCandCap = 1u << 16;
-constexpr u32 kHitCap = 1u << 16;
 constexpr u32 kMaxBatch = 256;
-
 
 u32 rows_of(u32 w) { return (MK_M_BITS + w - 1) / w; }
 u32 cols_of(u32 w) { return (1u << (w - 1)) + 1; }
 
-// Device buffer that grows on demand.
-struct dbuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    mk_status ensure(size_t bytes) {
-        if (bytes <= cap) return MK_OK;
-        if (p) {
-            hipError_t e = hipFree(p);
-            p = nullptr;
-            cap = 0;
-            if (e != hipSuccess) return fail(MK_ERR_DRIVER, "hipFree: %s", hipGetErrorString(e));
-        }
-        size_t want = std::max<size_t>(bytes, 256);
-        hipError_t e = hipMalloc(&p, want);
-        if (e != hipSuccess) return fail(MK_ERR_DRIVER, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
-        cap = want;
-        return MK_OK;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
+}  // namespace
 
-void key_to_words(const uint8_t key[32], u32 w[8]) {
+void mk_key_to_words(const uint8_t key[32], u32 w[8]) {
     for (int i = 0; i < 8; i++)
         w[i] = (u32)key[4 * i] | (u32)key[4 * i + 1] << 8 | (u32)key[4 * i + 2] << 16 | (u32)key[4 * i + 3] << 24;
 }
 
-void words_to_bytes(const u32 w[8], uint8_t out[32]) {
+void mk_words_to_bytes(const u32 w[8], uint8_t out[32]) {
     for (int i = 0; i < 8; i++)
         for (int k = 0; k < 4; k++) out[4 * i + k] = (u8)(w[i] >> (8 * k));
 }
-
-}  // namespace
-
-struct mk_gpu {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    mk_options opt{};
-    u32 cus = 0, grid = 0, rows = 0, cols = 0;
-    u64 lanes = 0, hbm = 0;
-    char name[64] = {};
-    u32* table = nullptr;
-    u64 table_bytes = 0;
-    u32* scratch = nullptr;
-    u64 scratch_bytes = 0;
-    mk_counters* ctr = nullptr;
-    mk_cand* cands = nullptr;
-    mk_hit* hits = nullptr;
-    mk_counters* h_ctr = nullptr;  // pinned
-    mk_hit* h_hits = nullptr;      // pinned
-    // per-call pools of the batched path
-    dbuf descs, masks, dfas, segs, stop;
-    std::vector<u8> staging;
-    // events around the last step's search and confirm kernels (on `stream`)
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    float last_search_ms = 0.f, last_confirm_ms = 0.f;
-};
 
 struct mk_launch {
     mk_gpu* g = nullptr;
@@ -120,18 +73,14 @@ struct mk_launch {
     u32* stop = nullptr;
 };
 
-namespace {
-
-// One step: clear the counters, run the search kernel over the segments, confirm the
-// candidates, and copy counters and exact hits back. Hits come back unsorted.
-mk_status run_step(mk_gpu* g, const mk_search_desc* descs, const u64* masks, const u16* dfas, const mk_seg* segs,
+mk_status mk_run_step(mk_gpu* g, const mk_search_desc* descs, const u64* masks, const u16* dfas, const mk_seg* segs,
                    u32 nsegs, u64 total_tiles, u32* stop, u32 nstop, std::vector<mk_hit>& out, u32& n_cands,
                    u32& n_hits, u32& overflow) {
-    MK_HIP(mk_k_reset(g->stream, g->ctr, stop, nstop));
+    MK_HIP(mk_k_reset(g->stream, g->ctr, stop, nstop, nullptr));
     MK_HIP(hipEventRecord(g->ev[0], g->stream));
     if (total_tiles > 0)
         MK_HIP(mk_k_search(g->stream, g->grid, g->table, g->opt.width, g->opt.batch, descs, segs, nsegs, total_tiles,
-                           masks, dfas, g->scratch, g->ctr, g->cands, kCandCap, stop));
+                           masks, dfas, g->scratch, g->ctr, g->cands, kCandCap, stop, nullptr, nullptr));
     MK_HIP(hipEventRecord(g->ev[1], g->stream));
     if (total_tiles > 0)
         MK_HIP(mk_k_confirm(g->stream, g->table, g->opt.width, descs, masks, dfas, g->ctr, g->cands, kCandCap, g->hits,
@@ -154,6 +103,8 @@ mk_status run_step(mk_gpu* g, const mk_search_desc* descs, const u64* masks, con
     return MK_OK;
 }
 
+namespace {
+
 void to_records(std::vector<mk_hit>& hs, mk_hit_rec* recs, u32 cap) {
     std::sort(hs.begin(), hs.end(), [](const mk_hit& a, const mk_hit& b) {
         return a.search != b.search ? a.search < b.search : a.index < b.index;
@@ -163,8 +114,8 @@ void to_records(std::vector<mk_hit>& hs, mk_hit_rec* recs, u32 cap) {
         recs[i].index = hs[i].index;
         recs[i].search = hs[i].search;
         recs[i].pad = 0;
-        words_to_bytes(hs[i].seed, recs[i].seed);
-        words_to_bytes(hs[i].pk, recs[i].pk);
+        mk_words_to_bytes(hs[i].seed, recs[i].seed);
+        mk_words_to_bytes(hs[i].pk, recs[i].pk);
     }
 }
 
@@ -410,7 +361,7 @@ mk_status mk_prepare(mk_gpu* g, const uint8_t key[32], int seed_source, const ui
     l->g = g;
     l->stop_on_hit = stop_on_hit ? 1u : 0u;
     mk_search_desc d{};
-    key_to_words(key, d.key);
+    mk_key_to_words(key, d.key);
     d.source = (u32)seed_source;
     d.mode = m.mode;
     d.len = m.len;
@@ -448,7 +399,7 @@ mk_status mk_launch_run(mk_launch* l, uint64_t base, uint32_t iters, mk_hit_rec*
     MK_HIP(hipMemcpyAsync(l->segs, &seg, sizeof seg, hipMemcpyHostToDevice, g->stream));
     std::vector<mk_hit> hs;
     u32 n_cands = 0, nh = 0, overflow = 0;
-    mk_status st = run_step(g, l->descs, l->masks, l->dfas, l->segs, 1, seg.tile_end, l->stop, 1, hs, n_cands, nh,
+    mk_status st = mk_run_step(g, l->descs, l->masks, l->dfas, l->segs, 1, seg.tile_end, l->stop, 1, hs, n_cands, nh,
                             overflow);
     if (st != MK_OK) return st;
     *n_hits = nh;
@@ -482,7 +433,7 @@ mk_status mk_derive(mk_gpu* g, const uint8_t key[32], int seed_source, uint64_t 
     if (n == 0) return MK_OK;
     MK_HIP(hipSetDevice(g->device));
     u32 kw[8];
-    key_to_words(key, kw);
+    mk_key_to_words(key, kw);
     const u64 chunk = 1ull << 22;
     u32 *dkey = nullptr, *dpk = nullptr, *dseed = nullptr;
     MK_HIP(hipMalloc((void**)&dkey, 32));
@@ -551,7 +502,7 @@ mk_status mk_batch_run(mk_gpu* g, const mk_search_spec* specs, uint32_t n, mk_hi
         if (st != MK_OK) return fail(st, "search %u: %s", i, g_last_error.c_str());
         mk_search_desc& d = descs[i];
         memset(&d, 0, sizeof d);
-        key_to_words(s.key, d.key);
+        mk_key_to_words(s.key, d.key);
         d.source = (u32)s.seed_source;
         d.mode = m.mode;
         d.len = m.len;
@@ -586,7 +537,7 @@ mk_status mk_batch_run(mk_gpu* g, const mk_search_spec* specs, uint32_t n, mk_hi
         MK_HIP(hipMemcpyAsync(g->segs.p, live.data(), sizeof(mk_seg) * live.size(), hipMemcpyHostToDevice, g->stream));
     std::vector<mk_hit> hs;
     u32 n_cands = 0, nh = 0, overflow = 0;
-    st = run_step(g, (const mk_search_desc*)g->descs.p, (const u64*)g->masks.p, (const u16*)g->dfas.p,
+    st = mk_run_step(g, (const mk_search_desc*)g->descs.p, (const u64*)g->masks.p, (const u16*)g->dfas.p,
                   (const mk_seg*)g->segs.p, (u32)live.size(), live.empty() ? 0 : tiles, (u32*)g->stop.p, n, hs,
                   n_cands, nh, overflow);
     if (st != MK_OK) return st;

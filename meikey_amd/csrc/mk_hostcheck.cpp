@@ -113,7 +113,7 @@ int hc_lane(const u32* table, u32 w, const u32 key[8], u32 source, u64 first, u3
         fe xn, xd;
         mk_mul_base_frac(table, w, s, yn, yd, xn, xd);
     };
-    mk_lane_tile(K, key, source, first, sp, mul, [&](u32 j, const u32 pk[8]) {
+    mk_lane_tile(K, key, source, first, sp, mul, [] {}, [&](u32 j, const u32 pk[8]) {
         memcpy(ys + 8 * j, pk, 32);
         cand[j] = mk_pk_match(pk, mt, true) ? 1 : 0;
     });

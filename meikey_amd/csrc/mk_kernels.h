@@ -41,6 +41,16 @@ struct mk_hit {
     u32 pk[8];
 };
 
+// Serving: the active request slots of a step. Tile t of the step goes to slot
+// slots[t % n_active]; the slot's next index (next_index[slot], device-resident) is
+// advanced atomically by each tile it receives.
+#define MK_SERVE_MAX_SLOTS 4096
+struct mk_serve_state {
+    u32 n_active;
+    u32 pad;
+    u32 slots[MK_SERVE_MAX_SLOTS];
+};
+
 // Launch-wide counters, cleared before each step.
 struct mk_counters {
     u64 queue;      // next tile to hand out

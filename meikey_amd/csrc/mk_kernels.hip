@@ -93,11 +93,15 @@ __device__ static u64 mk_uniform64(u64 v) {
 #define MK_SEARCH_WAVES 2
 #endif
 
+// Largest DFA that fits the 16 KB staging buffers (states x 65 x 2 bytes).
+#define MK_LDS_DFA_STATES 126
+
 extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
     mk_search_kernel(const u32* __restrict__ table, u32 w, u32 K, const mk_search_desc* __restrict__ descs,
                      const mk_seg* __restrict__ segs, u32 nsegs, u64 total_tiles, const u64* __restrict__ masks,
                      const u16* __restrict__ dfas, u32* __restrict__ scratch, mk_counters* __restrict__ ctr,
-                     mk_cand* __restrict__ cands, u32 cand_cap, u32* __restrict__ stop) {
+                     mk_cand* __restrict__ cands, u32 cand_cap, u32* __restrict__ stop,
+                     const mk_serve_state* __restrict__ serve, u64* __restrict__ next_index) {
     __shared__ u64 lmask[64];
     __shared__ uint4 stage[2 * 8 * 64];
     const u32 lane = threadIdx.x;
@@ -109,21 +113,36 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
         if (lane == 0) tile = atomicAdd((unsigned long long*)&ctr->queue, 1ull);
         tile = mk_uniform64(tile);
         if (tile >= total_tiles) break;
-        // segment holding this tile (binary search; segments are few and in order)
-        u32 lo = 0, hi = nsegs;
-        while (hi - lo > 1) {
-            const u32 mid = (lo + hi) >> 1;
-            if (segs[mid].tile_begin <= tile) lo = mid;
-            else hi = mid;
+        u32 search;
+        u64 tile_first;
+        if (serve != nullptr) {
+            // serving: round-robin over the active slots, indices from the slot's counter
+            const u32 n_active = serve->n_active;
+            if (n_active == 0) break;
+            search = serve->slots[tile % n_active];
+            if (__atomic_load_n(&stop[search], __ATOMIC_RELAXED) != 0) continue;
+            u64 f = 0;
+            if (lane == 0) f = atomicAdd((unsigned long long*)&next_index[search], 64ull * K);
+            tile_first = mk_uniform64(f);
+        } else {
+            // segment holding this tile (binary search; segments are few and in order)
+            u32 lo = 0, hi = nsegs;
+            while (hi - lo > 1) {
+                const u32 mid = (lo + hi) >> 1;
+                if (segs[mid].tile_begin <= tile) lo = mid;
+                else hi = mid;
+            }
+            const mk_seg sg = segs[lo];
+            search = sg.search;
+            tile_first = sg.first + (tile - sg.tile_begin) * 64ull * K;
         }
-        const mk_seg sg = segs[lo];
-        const mk_search_desc& dsc = descs[sg.search];
-        if (dsc.stop_on_hit && __atomic_load_n(&stop[sg.search], __ATOMIC_RELAXED) != 0) continue;
-        if (sg.search != cached) {
+        const mk_search_desc& dsc = descs[search];
+        if (dsc.stop_on_hit && __atomic_load_n(&stop[search], __ATOMIC_RELAXED) != 0) continue;
+        if (search != cached) {
             __syncthreads();
             lmask[lane] = masks[(u64)dsc.mask_off + lane];
             __syncthreads();
-            cached = sg.search;
+            cached = search;
         }
         mk_matcher mt;
         mt.mode = dsc.mode;
@@ -133,8 +152,8 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
         u32 key[8];
         MK_UNROLL
         for (int i = 0; i < 8; i++) key[i] = dsc.key[i];
-        const u64 first = sg.first + (tile - sg.tile_begin) * 64ull * K + (u64)lane * K;
-        const u32 search = sg.search, stop_on_hit = dsc.stop_on_hit;
+        const u64 first = tile_first + (u64)lane * K;
+        const u32 stop_on_hit = dsc.stop_on_hit;
         mk_spill sp;
         sp.base = scratch + gl;
         sp.stride = lanes;
@@ -142,7 +161,19 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
             fe xn, xd;
             mk_mul_base_frac_staged(table, w, s, stage, lane, yn, yd, xn, xd);
         };
-        mk_lane_tile(K, key, dsc.source, first, sp, mul, [&](u32 jj, const u32 pk[8]) {
+        // During the back-sweep the staging buffers are idle: a DFA of up to
+        // MK_LDS_DFA_STATES states is copied into them so its 43 dependent lookups per
+        // key hit LDS instead of L2.
+        auto pre_sweep = [&] {
+            if (mt.mode == MK_MODE_REGEX && mt.len <= MK_LDS_DFA_STATES) {
+                u16* ldfa = (u16*)stage;
+                const u16* src = dfas + dsc.dfa_off;
+                for (u32 i = lane; i < mt.len * 65; i += 64) ldfa[i] = src[i];
+                __syncthreads();
+                mt.dfa = ldfa;
+            }
+        };
+        mk_lane_tile(K, key, dsc.source, first, sp, mul, pre_sweep, [&](u32 jj, const u32 pk[8]) {
             const bool cand = mk_pk_match(pk, mt, true);
             const u64 bal = __ballot(cand);
             if (bal == 0) return;
@@ -205,7 +236,8 @@ extern "C" __global__ void __launch_bounds__(64)
 
 // Clears the launch counters and the first nstop stop flags with a kernel (a plain
 // fill rather than a memset node, so the step can be captured in a graph).
-extern "C" __global__ void mk_reset_kernel(mk_counters* ctr, u32* stop, u32 nstop) {
+// With `serve`, the stop flags of the step's active slots are cleared instead.
+extern "C" __global__ void mk_reset_kernel(mk_counters* ctr, u32* stop, u32 nstop, const mk_serve_state* serve) {
     const u32 i = blockIdx.x * 64 + threadIdx.x;
     if (i == 0) {
         ctr->queue = 0;
@@ -213,7 +245,11 @@ extern "C" __global__ void mk_reset_kernel(mk_counters* ctr, u32* stop, u32 nsto
         ctr->n_hits = 0;
         ctr->overflow = 0;
     }
-    if (stop != nullptr && i < nstop) stop[i] = 0;
+    if (serve != nullptr) {
+        if (i < serve->n_active && i < MK_SERVE_MAX_SLOTS) stop[serve->slots[i]] = 0;
+    } else if (stop != nullptr && i < nstop) {
+        stop[i] = 0;
+    }
 }
 
 // ------------------------------------------------------------ host launchers
@@ -244,16 +280,18 @@ hipError_t mk_k_derive_seeds(hipStream_t s, const u32* table, u32 w, const u32* 
     return hipGetLastError();
 }
 
-hipError_t mk_k_reset(hipStream_t s, mk_counters* ctr, u32* stop, u32 nstop) {
-    hipLaunchKernelGGL(mk_reset_kernel, dim3(nstop > 64 ? (nstop + 63) / 64 : 1), dim3(64), 0, s, ctr, stop, nstop);
+hipError_t mk_k_reset(hipStream_t s, mk_counters* ctr, u32* stop, u32 nstop, const mk_serve_state* serve) {
+    hipLaunchKernelGGL(mk_reset_kernel, dim3(nstop > 64 ? (nstop + 63) / 64 : 1), dim3(64), 0, s, ctr, stop, nstop,
+                       serve);
     return hipGetLastError();
 }
 
 hipError_t mk_k_search(hipStream_t s, u32 grid, const u32* table, u32 w, u32 K, const mk_search_desc* descs,
                        const mk_seg* segs, u32 nsegs, u64 total_tiles, const u64* masks, const u16* dfas,
-                       u32* scratch, mk_counters* ctr, mk_cand* cands, u32 cand_cap, u32* stop) {
+                       u32* scratch, mk_counters* ctr, mk_cand* cands, u32 cand_cap, u32* stop,
+                       const mk_serve_state* serve, u64* next_index) {
     hipLaunchKernelGGL(mk_search_kernel, dim3(grid), dim3(64), 0, s, table, w, K, descs, segs, nsegs, total_tiles,
-                       masks, dfas, scratch, ctr, cands, cand_cap, stop);
+                       masks, dfas, scratch, ctr, cands, cand_cap, stop, serve, next_index);
     return hipGetLastError();
 }
 

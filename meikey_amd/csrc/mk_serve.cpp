@@ -1,0 +1,406 @@
+// Serving: continuous batching of search requests on one device (no reference
+// counterpart; the reference runs one search per GPU, src/service/search.rs:129-192).
+//
+// Requests are queued on submit and admitted into a device slot when a slot is free and
+// their DFA table fits the DFA pool (the HBM budget). Each slot's descriptor, bitap
+// table, DFA table, next key index and stop flag live on the device. A step is one
+// launch of a fixed number of tiles dealt round-robin over the active slots: the search
+// kernel claims each tile's index range from the slot's device counter, stops a slot
+// once it has a candidate, and the confirm kernel turns candidates into exact hits.
+// After the step the host reads the counters, retires every slot with an exact hit
+// (its lowest-index hit of the step is the result) or with its candidate budget spent,
+// and admits waiting requests. The step -- slot-list upload, counter reset, search,
+// confirm, counter and next-index download -- is captured once as a hipGraph and
+// replayed; its kernels clear their own counters (no memset nodes).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "mk_internal.h"
+
+#define MK_MATCHER_FAIL(st, ...) (mk_fail((mk_status)(st), __VA_ARGS__), (st))
+#include "mk_matcher_host.h"
+
+#define MK_HIPS(call)                                                                                        \
+    do {                                                                                                     \
+        hipError_t e_ = (call);                                                                              \
+        if (e_ != hipSuccess)                                                                                \
+            return mk_fail(MK_ERR_DRIVER, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+namespace {
+
+struct pending {
+    u64 id;
+    mk_search_spec spec;
+    std::vector<u8> needle;  // owned copy (spec.needle points into it)
+    double submit;
+};
+
+struct slot_state {
+    bool used = false;
+    u64 request = 0;
+    u64 first = 0, count = 0;
+    size_t dfa_off = 0, dfa_n = 0;  // u16 units in the pool
+    double submit = 0, admit = 0;
+};
+
+// First-fit allocator over the DFA pool (u16 units).
+struct pool_alloc {
+    std::map<size_t, size_t> free_;  // offset -> size
+    void init(size_t n) {
+        free_.clear();
+        if (n) free_[0] = n;
+    }
+    bool take(size_t n, size_t& off) {
+        if (n == 0) {
+            off = 0;
+            return true;
+        }
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                off = it->first;
+                const size_t rest = it->second - n;
+                free_.erase(it);
+                if (rest) free_[off + n] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void give(size_t off, size_t n) {
+        if (n == 0) return;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+}  // namespace
+
+struct mk_server {
+    mk_gpu* g = nullptr;
+    mk_server_options opt{};
+    u64 step_tiles = 0;
+    // device
+    mk_search_desc* d_descs = nullptr;
+    u64* d_masks = nullptr;
+    u16* d_dfas = nullptr;
+    u32* d_stop = nullptr;
+    u64* d_next = nullptr;
+    mk_serve_state* d_serve = nullptr;
+    // pinned host
+    mk_serve_state* h_serve = nullptr;
+    u64* h_next = nullptr;
+    mk_counters* h_ctr = nullptr;
+    // graph of one step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // host state
+    std::vector<slot_state> slots;
+    std::deque<pending> queue;
+    std::vector<mk_server_result> done;
+    pool_alloc pool;
+    u64 next_id = 1;
+    std::chrono::steady_clock::time_point t0;
+    u64 steps = 0, keys = 0;
+};
+
+namespace {
+
+double now_s(const mk_server* s) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+// The step's device work on the server's stream (eager, or recorded into the graph).
+mk_status enqueue_step(mk_server* s) {
+    mk_gpu* g = s->g;
+    MK_HIPS(hipMemcpyAsync(s->d_serve, s->h_serve, sizeof(mk_serve_state), hipMemcpyHostToDevice, g->stream));
+    MK_HIPS(mk_k_reset(g->stream, g->ctr, s->d_stop, MK_SERVE_MAX_SLOTS, s->d_serve));
+    MK_HIPS(mk_k_search(g->stream, g->grid, g->table, g->opt.width, g->opt.batch, s->d_descs, nullptr, 0, s->step_tiles,
+                        s->d_masks, s->d_dfas, g->scratch, g->ctr, g->cands, kCandCap, s->d_stop, s->d_serve, s->d_next));
+    MK_HIPS(mk_k_confirm(g->stream, g->table, g->opt.width, s->d_descs, s->d_masks, s->d_dfas, g->ctr, g->cands, kCandCap,
+                         g->hits, kHitCap));
+    MK_HIPS(hipMemcpyAsync(s->h_ctr, g->ctr, sizeof(mk_counters), hipMemcpyDeviceToHost, g->stream));
+    MK_HIPS(hipMemcpyAsync(s->h_next, s->d_next, sizeof(u64) * s->opt.max_slots, hipMemcpyDeviceToHost, g->stream));
+    return MK_OK;
+}
+
+void finish(mk_server* s, u32 slot, int status, const mk_hit* hit, u64 tries) {
+    slot_state& st = s->slots[slot];
+    mk_server_result r;
+    memset(&r, 0, sizeof r);
+    r.request = st.request;
+    r.status = status;
+    r.tries = tries;
+    r.submit_s = st.submit;
+    r.admit_s = st.admit;
+    r.done_s = now_s(s);
+    if (hit) {
+        r.hit.index = hit->index;
+        r.hit.search = 0;
+        mk_words_to_bytes(hit->seed, r.hit.seed);
+        mk_words_to_bytes(hit->pk, r.hit.pk);
+    }
+    s->done.push_back(r);
+    s->pool.give(st.dfa_off, st.dfa_n);
+    st = slot_state();
+}
+
+// Move waiting requests into free slots while their DFA fits the pool.
+mk_status admit(mk_server* s) {
+    mk_gpu* g = s->g;
+    while (!s->queue.empty()) {
+        u32 slot = 0;
+        while (slot < s->opt.max_slots && s->slots[slot].used) slot++;
+        if (slot == s->opt.max_slots) break;
+        pending& p = s->queue.front();
+        matcher_host m;
+        if (build_matcher(p.spec.needle, p.spec.needle_len, p.spec.fold, p.spec.mode, m) != 0) {
+            mk_server_result r;
+            memset(&r, 0, sizeof r);
+            r.request = p.id;
+            r.status = -1;
+            r.submit_s = p.submit;
+            r.admit_s = r.done_s = now_s(s);
+            s->done.push_back(r);
+            s->queue.pop_front();
+            continue;
+        }
+        size_t off = 0;
+        if (!s->pool.take(m.dfa.size(), off)) break;  // wait for memory (FIFO order kept)
+        slot_state& st = s->slots[slot];
+        st.used = true;
+        st.request = p.id;
+        st.first = p.spec.first;
+        st.count = p.spec.count;
+        st.dfa_off = off;
+        st.dfa_n = m.dfa.size();
+        st.submit = p.submit;
+        st.admit = now_s(s);
+        mk_search_desc d;
+        memset(&d, 0, sizeof d);
+        mk_key_to_words(p.spec.key, d.key);
+        d.source = (u32)p.spec.seed_source;
+        d.mode = m.mode;
+        d.len = m.len;
+        d.stop_on_hit = 1;
+        d.mask_off = slot * 64;
+        d.dfa_off = (u32)off;
+        // synchronous copies from pageable memory: the sources go away when the request does
+        MK_HIPS(hipMemcpy(s->d_descs + slot, &d, sizeof d, hipMemcpyHostToDevice));
+        MK_HIPS(hipMemcpy(s->d_masks + (size_t)slot * 64, m.mask, sizeof m.mask, hipMemcpyHostToDevice));
+        if (!m.dfa.empty())
+            MK_HIPS(hipMemcpy(s->d_dfas + off, m.dfa.data(), m.dfa.size() * sizeof(u16), hipMemcpyHostToDevice));
+        const u64 first = p.spec.first;
+        MK_HIPS(hipMemcpy(s->d_next + slot, &first, sizeof first, hipMemcpyHostToDevice));
+        s->queue.pop_front();
+        (void)g;
+    }
+    return MK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mk_server_default_options(mk_server_options* out) {
+    if (!out) return;
+    out->max_slots = 1024;
+    out->use_graph = 1;
+    out->dfa_budget = 64ull << 20;
+    out->step_keys = 1ull << 26;
+}
+
+mk_status mk_server_open(mk_gpu* g, const mk_server_options* opt_in, mk_server** out) {
+    if (!g || !out) return mk_fail(MK_ERR_INVALID, "null pointer");
+    *out = nullptr;
+    mk_server_options opt;
+    mk_server_default_options(&opt);
+    if (opt_in) opt = *opt_in;
+    if (opt.max_slots < 1 || opt.max_slots > MK_SERVE_MAX_SLOTS)
+        return mk_fail(MK_ERR_OUT_OF_RANGE, "max_slots must be 1-%d (got %u)", MK_SERVE_MAX_SLOTS, opt.max_slots);
+    MK_HIPS(hipSetDevice(g->device));
+    mk_server* s = new mk_server();
+    s->g = g;
+    s->opt = opt;
+    const u64 tile_keys = 64ull * g->opt.batch;
+    s->step_tiles = std::max<u64>(g->grid, (opt.step_keys + tile_keys - 1) / tile_keys);
+    s->slots.resize(opt.max_slots);
+    const size_t dfa_units = std::max<size_t>(opt.dfa_budget / sizeof(u16), 65);
+    s->pool.init(dfa_units);
+    s->t0 = std::chrono::steady_clock::now();
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_descs, sizeof(mk_search_desc) * opt.max_slots);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_masks, sizeof(u64) * 64 * opt.max_slots);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_dfas, sizeof(u16) * dfa_units);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_stop, sizeof(u32) * MK_SERVE_MAX_SLOTS);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_next, sizeof(u64) * opt.max_slots);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_serve, sizeof(mk_serve_state));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_serve, sizeof(mk_serve_state), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_next, sizeof(u64) * opt.max_slots, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_ctr, sizeof(mk_counters), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        mk_server_close(s);
+        return mk_fail(MK_ERR_DRIVER, "server allocation: %s", hipGetErrorString(e));
+    }
+    memset(s->h_serve, 0, sizeof(mk_serve_state));
+    std::vector<u32> zeros(MK_SERVE_MAX_SLOTS, 0);
+    e = hipMemcpy(s->d_stop, zeros.data(), sizeof(u32) * MK_SERVE_MAX_SLOTS, hipMemcpyHostToDevice);
+    if (e == hipSuccess && opt.use_graph) {
+        e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            mk_status st = enqueue_step(s);
+            hipError_t e2 = hipStreamEndCapture(g->stream, &s->graph);
+            if (st != MK_OK) {
+                mk_server_close(s);
+                return st;
+            }
+            e = e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0);
+    }
+    if (e != hipSuccess) {
+        mk_server_close(s);
+        return mk_fail(MK_ERR_DRIVER, "server graph: %s", hipGetErrorString(e));
+    }
+    *out = s;
+    return MK_OK;
+}
+
+double mk_server_now(const mk_server* s) { return s ? now_s(s) : 0.0; }
+
+mk_status mk_server_submit(mk_server* s, const mk_search_spec* spec, uint64_t* request) {
+    if (!s || !spec) return mk_fail(MK_ERR_INVALID, "null pointer");
+    pending p;
+    p.id = s->next_id++;
+    p.spec = *spec;
+    size_t bytes = spec->mode == MK_MODE_REGEX ? (size_t)spec->needle_len * 130 : spec->needle_len;
+    if (bytes && !spec->needle) return mk_fail(MK_ERR_INVALID, "null needle");
+    p.needle.assign(spec->needle, spec->needle + bytes);
+    p.submit = now_s(s);
+    s->queue.push_back(std::move(p));
+    s->queue.back().spec.needle = s->queue.back().needle.data();
+    if (request) *request = s->queue.back().id;
+    return MK_OK;
+}
+
+mk_status mk_server_step(mk_server* s, uint32_t* active_out, uint32_t* queued_out) {
+    if (!s) return mk_fail(MK_ERR_INVALID, "null pointer");
+    mk_gpu* g = s->g;
+    MK_HIPS(hipSetDevice(g->device));
+    mk_status st = admit(s);
+    if (st != MK_OK) return st;
+    u32 n = 0;
+    for (u32 i = 0; i < s->opt.max_slots; i++)
+        if (s->slots[i].used) s->h_serve->slots[n++] = i;
+    s->h_serve->n_active = n;
+    if (active_out) *active_out = n;
+    if (queued_out) *queued_out = (u32)s->queue.size();
+    if (n == 0) return MK_OK;
+    if (s->exec) {
+        MK_HIPS(hipGraphLaunch(s->exec, g->stream));
+    } else {
+        st = enqueue_step(s);
+        if (st != MK_OK) return st;
+    }
+    MK_HIPS(hipStreamSynchronize(g->stream));
+    s->steps++;
+    const u32 nh = std::min(s->h_ctr->n_hits, kHitCap);
+    std::vector<mk_hit> hits(nh);
+    if (nh) {
+        MK_HIPS(hipMemcpy(hits.data(), g->hits, sizeof(mk_hit) * nh, hipMemcpyDeviceToHost));
+    }
+    // lowest-index exact hit of each slot in this step
+    std::map<u32, mk_hit> best;
+    for (const mk_hit& h : hits) {
+        if (h.search >= s->opt.max_slots || !s->slots[h.search].used) continue;
+        auto it = best.find(h.search);
+        if (it == best.end() || h.index < it->second.index) best[h.search] = h;
+    }
+    for (u32 i = 0; i < n; i++) {
+        const u32 slot = s->h_serve->slots[i];
+        const u64 tries = s->h_next[slot] - s->slots[slot].first;
+        auto it = best.find(slot);
+        if (it != best.end()) {
+            finish(s, slot, 1, &it->second, tries);
+        } else if (s->slots[slot].count && tries >= s->slots[slot].count) {
+            finish(s, slot, 0, nullptr, tries);
+        }
+    }
+    return MK_OK;
+}
+
+mk_status mk_server_poll(mk_server* s, mk_server_result* out, uint32_t cap, uint32_t* n) {
+    if (!s || !n || (cap && !out)) return mk_fail(MK_ERR_INVALID, "null pointer");
+    const u32 k = (u32)std::min<size_t>(cap, s->done.size());
+    for (u32 i = 0; i < k; i++) out[i] = s->done[i];
+    s->done.erase(s->done.begin(), s->done.begin() + k);
+    *n = k;
+    return MK_OK;
+}
+
+mk_status mk_server_cancel(mk_server* s, uint64_t request) {
+    if (!s) return mk_fail(MK_ERR_INVALID, "null pointer");
+    for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
+        if (it->id == request) {
+            mk_server_result r;
+            memset(&r, 0, sizeof r);
+            r.request = request;
+            r.status = -1;
+            r.submit_s = it->submit;
+            r.admit_s = r.done_s = now_s(s);
+            s->done.push_back(r);
+            s->queue.erase(it);
+            return MK_OK;
+        }
+    for (u32 i = 0; i < s->opt.max_slots; i++)
+        if (s->slots[i].used && s->slots[i].request == request) {
+            finish(s, i, -1, nullptr, s->h_next[i] - s->slots[i].first);
+            return MK_OK;
+        }
+    return mk_fail(MK_ERR_INVALID, "no request %llu", (unsigned long long)request);
+}
+
+mk_status mk_server_stats(const mk_server* s, uint64_t* steps, uint64_t* step_keys) {
+    if (!s) return mk_fail(MK_ERR_INVALID, "null pointer");
+    if (steps) *steps = s->steps;
+    if (step_keys) *step_keys = s->step_tiles * 64ull * s->g->opt.batch;
+    return MK_OK;
+}
+
+mk_status mk_server_close(mk_server* s) {
+    if (!s) return MK_OK;
+    (void)hipSetDevice(s->g->device);
+    (void)hipStreamSynchronize(s->g->stream);
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    if (s->d_descs) (void)hipFree(s->d_descs);
+    if (s->d_masks) (void)hipFree(s->d_masks);
+    if (s->d_dfas) (void)hipFree(s->d_dfas);
+    if (s->d_stop) (void)hipFree(s->d_stop);
+    if (s->d_next) (void)hipFree(s->d_next);
+    if (s->d_serve) (void)hipFree(s->d_serve);
+    if (s->h_serve) (void)hipHostFree(s->h_serve);
+    if (s->h_next) (void)hipHostFree(s->h_next);
+    if (s->h_ctr) (void)hipHostFree(s->h_ctr);
+    delete s;
+    return MK_OK;
+}
+
+}  // extern "C"

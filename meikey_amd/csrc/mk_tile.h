@@ -28,10 +28,11 @@ struct mk_spill {
 };
 
 // Keys first .. first + K - 1: seed -> scalar -> (yn, yd) by mul(s, yn, yd); one
-// inversion of the product of the yd; then, from the last key to the first, y = yn / yd
-// is compressed (x sign bit left 0) and on_key(j, pk_words) is called.
-template <class M, class F>
-MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const mk_spill& sp, M mul, F on_key) {
+// inversion of the product of the yd; then pre_sweep(); then, from the last key to the
+// first, y = yn / yd is compressed (x sign bit left 0) and on_key(j, pk_words) is called.
+template <class M, class P, class F>
+MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const mk_spill& sp, M mul, P pre_sweep,
+                         F on_key) {
     u32 upper[8];
     fe pre;
     MK_NO_UNROLL
@@ -47,6 +48,7 @@ MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const m
         sp.store(3 * j + 2, pre);
     }
     fe inv = fe_invert(pre);
+    pre_sweep();
     // 1/yd_j = inv * pre_(j-1) with inv = 1/pre_j; then inv <- inv * yd_j = 1/pre_(j-1)
     MK_NO_UNROLL
     for (u32 jj = K; jj-- > 0;) {

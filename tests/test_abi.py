@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared():
     text = open(os.path.join(ROOT, "include", "meikey_hip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:mk_status|void|uint64_t|int)\s+(mk_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:mk_status|void|uint64_t|uint32_t|int|size_t|double)\s+(mk_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_lists_entry_points():

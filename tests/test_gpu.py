@@ -198,3 +198,184 @@ def test_config1_full_hit_set(gpu, oracle):
     for h in hits:
         assert "".join(T.hay(h.pk)).lower().find(needle.decode()) >= 0
     say(f"config1 hits={len(want)}")
+
+
+@pytest.mark.parametrize("text,cs", [
+    ("ab", True), ("^[A-P]a|z$", False), ("(ab){2,}", True), ("\\b9", True), ("[0-9][A-F]x", False),
+    ("Q.$", True), ("[a-z]{2}[0-9]$", True), ("^[A-D][0-9]", True), ("a.b", False),
+])
+def test_regex_search_vs_oracle(gpu, oracle, text, cs):
+    """DFA matching in the search kernel (LDS-resident table, sign fork at tail position
+    42) against the oracle's DFA walk, and every hit against Python re on the real key."""
+    import re
+    p = M.Pattern(text, cs, M.MODE_REGEX)
+    key = random.Random(hash((text, cs)) & 0xFFFF).randbytes(32)
+    first = 12345
+    hits, tries = gpu.batch_run([p.spec(key, first, 1 << 19)])
+    table = p.needle()[0]
+    want = oracle.search(key, 0, first, tries[0], table, cs, M.MODE_REGEX)
+    assert [h.index for h in hits] == want
+    rx = re.compile(text, re.ASCII | (0 if cs else re.IGNORECASE))
+    for h in hits[:50]:
+        assert rx.search(T.hay(h.pk)) is not None
+
+
+def config4_specs(count, rng_seed=4):
+    """Config 4: 64 searches with master keys from PRNG seeds 1000..1063, kinds 25% each:
+    case-insensitive substring (3-7), case-sensitive substring, suffix (3-6), regex with
+    at most 64 DFA states (random anchored / unanchored)."""
+    rng = random.Random(rng_seed)
+    alpha = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
+    pats = []
+    for i in range(64):
+        kind = i % 4
+        if kind == 0:
+            p = M.Pattern("".join(rng.choice(alpha) for _ in range(rng.randint(3, 7))), False)
+        elif kind == 1:
+            p = M.Pattern("".join(rng.choice(alpha) for _ in range(rng.randint(3, 7))), True)
+        elif kind == 2:
+            p = M.Pattern("".join(rng.choice(alpha) for _ in range(rng.randint(3, 6))), rng.random() < 0.5,
+                          M.MODE_SUFFIX)
+        else:
+            while True:
+                body = "".join(rng.choice(["[A-Z]", "[0-9]", "[a-f]", rng.choice(alpha[:62]), "."])
+                               for _ in range(rng.randint(2, 4)))
+                text = rng.choice(["^", "", ""]) + body + rng.choice(["$", "", ""])
+                try:
+                    p = M.Pattern(text, True, M.MODE_REGEX)
+                except M.PatternError:
+                    continue
+                # at most 64 DFA states, and rare enough to be a search (>= 2^10 expected tries)
+                if p.dfa_states <= 64 and p.probability <= 2.0 ** -10:
+                    break
+        pats.append(p)
+    keys = [random.Random(1000 + i).randbytes(32) for i in range(64)]
+    return [p.spec(k, 0, count) for p, k in zip(pats, keys)], pats, keys
+
+
+def test_config4_batched_equals_single_and_oracle(gpu, oracle):
+    """64 concurrent searches of mixed kinds in one launch: per-search hit sets equal
+    to one search per launch over the full 2^22 range, and to the oracle over the
+    first 2^15 candidates of every search."""
+    n = 1 << 22
+    specs, pats, keys = config4_specs(n)
+    say("config4 batched")
+    batched, tries = gpu.batch_run(specs)
+    assert all(t == n for t in tries)
+    by_search = {i: [] for i in range(64)}
+    for h in batched:
+        by_search[h.search].append(h.index)
+    for i, s in enumerate(specs):
+        single, _ = gpu.batch_run([s])
+        assert by_search[i] == [h.index for h in single], i
+        sub = 1 << 15
+        want = oracle.search(keys[i], 0, 0, sub, s["needle"], not s["fold"], s["mode"])
+        assert [x for x in by_search[i] if x < sub] == want, i
+    say(f"config4 hits={len(batched)}")
+
+
+def _served_checks(oracle, spec, r):
+    """A served result is the first exact hit at or after `first`: the oracle finds the
+    same lowest hit in the range the request covered, and the key and seed re-derive."""
+    assert r.status == 1
+    assert r.tries > 0 and r.index >= spec["first"]
+    want = oracle.search(spec["key"], spec["seed_source"], spec["first"], r.index - spec["first"] + 1,
+                         spec["needle"], not spec["fold"], spec["mode"])
+    assert want and want[0] == r.index
+    assert oracle.derive(spec["key"], spec["seed_source"], r.index, 1) == r.pk
+    assert oracle.seed(spec["key"], spec["seed_source"], r.index) == r.seed
+
+
+def _serve_specs(n, seed):
+    rng = random.Random(seed)
+    alpha = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
+    out = []
+    for i in range(n):
+        kind = i % 4
+        L = rng.randint(2, 3)
+        text = "".join(rng.choice(alpha) for _ in range(L))
+        if kind == 0:
+            p = M.Pattern(text, False)
+        elif kind == 1:
+            p = M.Pattern(text, True)
+        elif kind == 2:
+            p = M.Pattern(text[:2], True, M.MODE_SUFFIX)
+        else:
+            p = M.Pattern(rng.choice(["[0-9]", "[A-F]", "a"]) + rng.choice(alpha[:62]), True, M.MODE_REGEX)
+        out.append(p.spec(random.Random(seed * 1000 + i).randbytes(32), rng.getrandbits(48), 0,
+                          seed_source=i % 2))
+    return out
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_server_results_alone_and_in_the_mix(gpu, oracle, use_graph):
+    """Continuous batching: requests submitted while others run; each result equals the
+    result of the same request served alone, and the oracle confirms it."""
+    specs = _serve_specs(24, 7)
+    with M.Server(gpu, max_slots=8, use_graph=use_graph, step_keys=1 << 22) as srv:
+        ids = {}
+        results = {}
+        i = 0
+        steps = 0
+        while len(results) < len(specs):
+            # two arrivals per step; at most 8 slots, so the rest queue
+            for _ in range(2):
+                if i < len(specs):
+                    ids[srv.submit(specs[i])] = i
+                    i += 1
+            srv.step()
+            steps += 1
+            for r in srv.poll():
+                results[ids[r.request]] = r
+            assert steps < 2000
+    for k, s in enumerate(specs):
+        _served_checks(oracle, s, results[k])
+    # alone
+    for k in (0, 5, 11, 23):
+        with M.Server(gpu, max_slots=4, use_graph=use_graph, step_keys=1 << 22) as srv:
+            rid = srv.submit(specs[k])
+            got = []
+            while not got:
+                srv.step()
+                got = srv.poll()
+            assert got[0].request == rid and got[0].index == results[k].index and got[0].pk == results[k].pk
+
+
+def test_server_graph_replays_match_eager(gpu):
+    """The captured step graph, replayed many times with host work in between, gives the
+    same per-request results as enqueuing each step."""
+    specs = _serve_specs(16, 9)
+    out = {}
+    for use_graph in (True, False):
+        with M.Server(gpu, max_slots=16, use_graph=use_graph, step_keys=1 << 21) as srv:
+            ids = {srv.submit(s): n for n, s in enumerate(specs)}
+            res = {}
+            while len(res) < len(specs):
+                srv.step()
+                for r in srv.poll():
+                    res[ids[r.request]] = (r.status, r.index, r.pk)
+            out[use_graph] = res
+    assert out[True] == out[False]
+
+
+def test_server_budget_and_cancel(gpu):
+    """Admission waits for DFA-pool memory; cancelled requests end with status -1; a
+    request with a candidate budget ends with status 0 when it is spent."""
+    big = M.Pattern("[A-Z]{3}[0-9]zz$", True, M.MODE_REGEX)
+    assert big.dfa_states > 1
+    table_bytes = len(big.needle()[0])
+    with M.Server(gpu, max_slots=8, use_graph=True, dfa_budget=table_bytes + 16, step_keys=1 << 20) as srv:
+        a = srv.submit(big.spec(b"\x01" * 32, 0, 1 << 21))
+        b = srv.submit(big.spec(b"\x02" * 32, 0, 1 << 21))
+        active, queued = srv.step()
+        assert (active, queued) == (1, 1)  # only one DFA fits the budget
+        srv.cancel(b)
+        done = {}
+        while a not in done:
+            srv.step()
+            for r in srv.poll():
+                done[r.request] = r
+        assert done[b].status == -1
+        assert done[a].status in (0, 1)
+        if done[a].status == 0:
+            assert done[a].tries >= 1 << 21
