@@ -7,14 +7,15 @@ A step is one search launch over 2^28 consecutive candidate indices; the default
 timed steps cover the full 2^32 range once.
 
 Reports one JSON line: keys/s (value), the search kernel's fraction of the integer
-VALU multiply roofline (timed with HIP events on the library's stream), and the CPU
-oracle's rate on the host cores over a bounded sample (which also checks the GPU's
-public keys of that sample bit for bit).
+VALU multiply roofline (timed with HIP events on the library's stream), the 64-search
+batched launch against the single search and against one search per launch, the
+serving run (Poisson arrivals, p50/p99 time to first hit), and the CPU oracle's rate on
+the host cores over a bounded sample (which also checks the GPU's public keys of that
+sample, and every key the server returned, bit for bit).
 """
 import argparse
 import json
 import os
-import random
 import sys
 import time
 
@@ -22,7 +23,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "secure-mode keys/s on 1 MI355X (single vs 64-search batched); p99 time-to-hit"
-ALPHABET = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
 STEP_KEYS = 1 << 28
 # MI355X integer VALU peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, one 32x32->64
 # multiply-add (v_mad_u64_u32) per lane per clock.
@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--blocks", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-search batched leg")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="serving run length (0 = skip)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -58,10 +60,10 @@ def main():
         sys.exit(2)
 
     import meikey_amd as M
+    from meikey_amd import workloads as W
 
-    rng = random.Random(2)
-    key = rng.randbytes(32)
-    needle = "".join(rng.choice(ALPHABET) for _ in range(6)).encode()
+    key, needle = W.single()
+    needle = needle.encode()
     gpu = M.Gpu(0, width=args.width, batch=args.batch, blocks=args.blocks)
     info = gpu.info()
     print(f"# device {info.name.decode()} CUs={info.compute_units} grid={info.grid_waves} waves "
@@ -103,10 +105,111 @@ def main():
                      "kernel": "mk_search_kernel", "kernel_ms": avg_ms,
                      "algorithmic": f"{mults:.2f} field mults/key x 64 32x32-bit products"},
     }
+    if not args.no_batch:
+        out["batched"] = batched_leg(gpu, rate)
+        out["batched"]["kernel_vs_single_kernel"] = out["batched"]["kernel_keys_per_s"] / (STEP_KEYS / (avg_ms * 1e-3))
+    served = []
+    if args.serve_seconds > 0:
+        out["serving"], served = serving_leg(gpu, args.serve_seconds)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(gpu, key, needle)
+        out["cpu_baseline"] = cpu_baseline(gpu, key, needle, served)
     gpu.close()
     print(json.dumps(out), flush=True)
+
+
+def batched_leg(gpu, single_rate):
+    """64 searches of mixed kinds (substring either case, suffix, regex DFA), 2^26
+    candidates each, in one launch; then the same 64 searches one launch each."""
+    from meikey_amd import workloads as W
+    specs, pats, keys = W.batched(1 << 26)
+    warm = W.batched(1 << 20)[0]
+    gpu.batch_run(warm)
+    t0 = time.perf_counter()
+    hits, tries = gpu.batch_run(specs)
+    dt = time.perf_counter() - t0
+    batched_ms = gpu.last_timing()[0]
+    total = sum(tries)
+    t0 = time.perf_counter()
+    single_hits = []
+    single_ms = 0.0
+    for s in specs:
+        h, _ = gpu.batch_run([s])
+        single_ms += gpu.last_timing()[0]
+        single_hits.append([x.index for x in h])
+    dt1 = time.perf_counter() - t0
+    by = [[] for _ in specs]
+    for h in hits:
+        by[h.search].append(h.index)
+    mismatched = sum(1 for a, b in zip(by, single_hits) if a != b)
+    rate = total / dt
+    print(f"# batched 64 searches: {rate / 1e9:.3f} G keys/s, one per launch {total / dt1 / 1e9:.3f} G keys/s, "
+          f"{len(hits)} hits, {mismatched} searches differ", file=sys.stderr, flush=True)
+    return {"value": rate, "unit": "keys/s", "searches": len(specs), "candidates": total,
+            "vs_single": rate / single_rate, "one_per_launch": total / dt1,
+            "vs_one_per_launch": dt1 / dt, "hits": len(hits), "mismatched_searches": mismatched,
+            "kernel_keys_per_s": total / (batched_ms * 1e-3),
+            "one_per_launch_kernel_keys_per_s": total / (single_ms * 1e-3),
+            "kinds": "16 substring case-insensitive 3-7, 16 substring case-sensitive 3-7, 16 suffix 3-6, "
+                     "16 regex (<= 64 DFA states)"}
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = min(len(xs) - 1, max(0, int(round(q * (len(xs) - 1)))))
+    return xs[k]
+
+
+def serving_leg(gpu, seconds):
+    """Poisson arrivals at 50/s for `seconds`, pattern length 3-6, kinds mixed 25% each,
+    every request stopping at its first hit. Time to first hit = done - submit. Requests
+    unfinished when the arrivals stop are cancelled and counted with their elapsed time,
+    a lower bound (censored); p50/p99 are over all submitted requests."""
+    import meikey_amd as M
+    from meikey_amd import workloads as W
+    reqs = W.serving(50.0, seconds)
+    srv = M.Server(gpu, max_slots=4096, use_graph=True, dfa_budget=256 << 20, step_keys=1 << 25,
+                   active_per_step=64)
+    ids, results, i = {}, {}, 0
+    t0 = srv.now()
+    steps = 0
+    while True:
+        now = srv.now() - t0
+        while i < len(reqs) and reqs[i].arrival_s <= now:
+            ids[srv.submit(reqs[i].spec())] = i
+            i += 1
+        if now >= seconds:
+            break
+        srv.step()
+        steps += 1
+        for r in srv.poll():
+            results[ids[r.request]] = r
+    end = srv.now() - t0
+    for rid, k in ids.items():
+        if k not in results:
+            srv.cancel(rid)
+    for r in srv.poll():
+        results[ids[r.request]] = r
+    _, step_keys = srv.stats()
+    srv.close()
+    found = [r for r in results.values() if r.status == 1]
+    ttfh = [r.done_s - r.submit_s for r in found]
+    censored = [end - (r.submit_s - t0) for r in results.values() if r.status != 1]
+    everything = ttfh + censored
+    tries = sum(r.tries for r in results.values())
+    p99 = percentile(everything, 0.99)
+    out = {"requests": len(ids), "found": len(found), "unfinished": len(censored), "seconds": end,
+           "steps": steps, "step_keys": step_keys, "keys_per_s": tries / end,
+           "p50_time_to_first_hit_s": percentile(everything, 0.5), "p99_time_to_first_hit_s": p99,
+           "p99_censored": bool(censored) and p99 is not None and p99 >= min(censored),
+           "p50_found_only_s": percentile(ttfh, 0.5), "p99_found_only_s": percentile(ttfh, 0.99),
+           "rate_per_s": 50.0, "policy": "least attained service, 64 requests per step, 2^25 candidates per step",
+           "censoring": "requests unfinished at the end count with their elapsed time (lower bound)"}
+    print(f"# serving: {len(found)}/{len(ids)} found, p50 {out['p50_time_to_first_hit_s']:.3f} s, "
+          f"p99 {p99:.3f} s, {tries / end / 1e9:.3f} G keys/s", file=sys.stderr, flush=True)
+    served = [(reqs[k], r) for k, r in results.items() if r.status == 1]
+    return out, served
 
 
 def measured_traffic(width: int, batch: int, blocks: int):
@@ -137,10 +240,10 @@ def copy_calibration(nbytes: int) -> None:
     print(json.dumps({"copy_bytes": nbytes, "dispatches": 3}), flush=True)
 
 
-def cpu_baseline(gpu, key, needle):
+def cpu_baseline(gpu, key, needle, served=()):
     """The CPU oracle (a scalar C port of the reference's per-key algorithm) on this
     host's cores over a bounded sample of the same workload; the GPU's public keys of
-    the same sample are checked against it bit for bit."""
+    the same sample, and every key the server returned, are checked against it."""
     from oracle import oracle as O
 
     cores = min(os.cpu_count() or 1, 16)
@@ -154,7 +257,13 @@ def cpu_baseline(gpu, key, needle):
     for f in firsts:
         if gpu.derive(key, 0, f, per) != O.derive(key, 0, f, per, threads=cores):
             mismatches += 1
+    bad_served = 0
+    for req, r in served:
+        pk = O.derive(req.key, 0, r.index, 1)
+        if pk != r.pk or O.seed(req.key, 0, r.index) != r.seed or not req.pattern.is_match(O.public_b64(pk)):
+            bad_served += 1
     return {"value": chunks * per / dt, "unit": "keys/s", "cores": cores, "kind": "port",
+            "served_keys_checked": len(served), "served_keys_wrong": bad_served,
             "sample": f"{chunks} x {per} consecutive candidates spread over the 2^32 range, derive + suffix match "
                       f"with the C oracle; GPU public keys of the same 2^20 candidates compared: "
                       f"{'all equal' if mismatches == 0 else f'{mismatches} chunks differ'}",

@@ -168,16 +168,19 @@ mk_status mk_last_timing(const mk_gpu* g, float* search_ms, float* confirm_ms);
 /* ------------------------------------------------------------------------------------
  * Serving: continuous batching of search requests (no reference counterpart). Requests
  * are admitted into device slots as slots and DFA-pool memory free up; each step is one
- * launch over a fixed number of tiles dealt round-robin across the active requests, whose
- * next key index and stop flag live on the device; the step is replayed from a captured
- * hipGraph. A request ends at its first exact hit (its lowest index found in that step),
+ * launch over a fixed number of tiles dealt round-robin across the requests served in that
+ * step -- the resident ones with the least attained service -- whose next key index and stop
+ * flag live on the device; the step is replayed from a captured hipGraph. A request ends at its first exact hit (its lowest index found in that step),
  * when `count` candidates are spent (count 0 = unbounded), or when cancelled. The server
  * uses the mk_gpu's stream and buffers: make no other call on that mk_gpu while it is open. */
 typedef struct {
-    uint32_t max_slots;  /* requests on the device at once, 1..4096 */
-    uint32_t use_graph;  /* 1: replay the captured step graph; 0: enqueue each step */
-    uint64_t dfa_budget; /* HBM bytes for the admitted requests' DFA tables */
-    uint64_t step_keys;  /* candidates per step (rounded up to whole tiles, at least one per wave) */
+    uint32_t max_slots;       /* requests resident on the device at once, 1..4096 */
+    uint32_t use_graph;       /* 1: replay the captured step graph; 0: enqueue each step */
+    uint64_t dfa_budget;      /* HBM bytes for the admitted requests' DFA tables */
+    uint64_t step_keys;       /* candidates per step (rounded up to whole tiles, at least one per wave) */
+    uint32_t active_per_step; /* resident requests served per step, the ones with the fewest
+                                 candidates searched so far (least attained service); 0 = all */
+    uint32_t pad;
 } mk_server_options;
 
 typedef struct {
@@ -240,6 +243,67 @@ double mk_pattern_probability(const mk_pattern* p);
 uint32_t mk_pattern_dfa_states(const mk_pattern* p);
 /* Pattern::is_match on the 68-character public-key base64 (1 / 0, -1 on bad input). */
 int mk_pattern_is_match(const mk_pattern* p, const char* b64);
+
+/* ------------------------------------------------------------------------------------
+ * Searcher::run for one GPU (src/service/search.rs:34-40, gpu.rs:999-1012) driving the
+ * Engine through search_with (gpu.rs:917-968): a fresh random key, then launches over
+ * consecutive index ranges whose iteration count adapts toward `target_launch_s` per
+ * launch (x0.5..x4 per step, 1..2^16), until an exact hit (not in bench mode), the cancel
+ * flag, or the time limit. */
+typedef struct {
+    uint64_t tries;
+    double elapsed_s;
+} mk_stats; /* Stats (src/model/search.rs:84-94) */
+
+typedef void (*mk_progress_fn)(void* user, const mk_stats* stats);
+
+typedef struct {
+    const mk_pattern* pattern;      /* NULL: no pattern (matches nothing; benchmark) */
+    int32_t seed_source;            /* MK_SEED_CHACHA20 / MK_SEED_COUNTER */
+    uint32_t bench;                 /* 1: keep searching through hits (--bench) */
+    double time_limit_s;            /* 0: none */
+    double progress_interval_s;     /* progress callback period (0: 5 s, model/search.rs:76) */
+    double target_launch_s;         /* per-launch target (0: 0.2 s, gpu.rs:427) */
+    const volatile int32_t* cancel; /* optional; non-zero stops the search between launches */
+    const uint8_t* key;             /* optional fixed 32-byte key (tests); NULL = fresh OS randomness */
+} mk_search_config;
+
+typedef struct {
+    int32_t found; /* 1: index / seed / pk hold the hit */
+    uint32_t pad;
+    mk_stats stats;
+    uint64_t launches;
+    uint64_t index;
+    uint8_t key[32]; /* the search key that was used */
+    uint8_t seed[32];
+    uint8_t pk[32];
+} mk_outcome;
+
+mk_status mk_search(mk_gpu* g, const mk_search_config* cfg, mk_progress_fn progress, void* user, mk_outcome* out);
+
+/* Autotuner (gpu.rs:1014-1113 fastest / fastest_staged): each candidate is opened (table
+ * build + self-check; a failing setting is reported with its status and skipped, like the
+ * reference's "Rejected" list) and measured as gpu.rs:889-911 does (counter seeds, needle
+ * "meikey" case-insensitive, three ~100 ms launches after a sizing launch). */
+typedef struct {
+    mk_options options;
+    double keys_per_s;
+    int32_t status; /* MK_OK, or the error that rejected the setting */
+    uint32_t pad;
+} mk_tune_result;
+
+mk_status mk_fastest(int ordinal, const mk_options* candidates, uint32_t n, mk_tune_result* results, mk_options* best);
+/* Stage 1: widths 16/20/22/24; stage 2: 32/64/128/256 keys per lane; stage 3: 1 or 2 waves
+ * per SIMD. results (cap entries) receive every measured setting; *n the count. */
+mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* results, uint32_t cap, uint32_t* n);
+
+/* OpenSSH text of a key: the authorized_keys line "ssh-ed25519 <b64>[ <comment>]"
+ * (key.rs:132-139) and the unencrypted openssh-key-v1 private key (key_file.rs:276-320;
+ * checkint is the random check word). Comments may not contain line breaks (key.rs:64-75).
+ * *len receives the text length; out (cap bytes, may be NULL to query) gets the text. */
+mk_status mk_openssh_public(const uint8_t pk[32], const char* comment, char* out, size_t cap, size_t* len);
+mk_status mk_openssh_private(const uint8_t seed[32], const uint8_t pk[32], const char* comment, uint32_t checkint,
+                             char* out, size_t cap, size_t* len);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,7 @@ EXPORTS = (
     "mk_pattern_probability", "mk_pattern_dfa_states", "mk_pattern_is_match",
     "mk_server_default_options", "mk_server_open", "mk_server_submit", "mk_server_step", "mk_server_poll",
     "mk_server_cancel", "mk_server_stats", "mk_server_now", "mk_server_close",
+    "mk_search", "mk_openssh_public", "mk_openssh_private", "mk_fastest", "mk_fastest_staged",
 )
 
 PATTERN_ERRORS = {1: "Empty", 2: "InvalidChar", 3: "TooLong", 4: "Regex", 5: "TooComplex", 6: "MatchesEmpty",
@@ -80,13 +81,38 @@ class SearchSpec(ctypes.Structure):
 
 class ServerOptions(ctypes.Structure):
     _fields_ = [("max_slots", ctypes.c_uint32), ("use_graph", ctypes.c_uint32), ("dfa_budget", ctypes.c_uint64),
-                ("step_keys", ctypes.c_uint64)]
+                ("step_keys", ctypes.c_uint64), ("active_per_step", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
 class ServerResult(ctypes.Structure):
     _fields_ = [("request", ctypes.c_uint64), ("status", ctypes.c_int32), ("pad", ctypes.c_uint32),
                 ("tries", ctypes.c_uint64), ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
                 ("done_s", ctypes.c_double), ("hit", HitRec)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("options", Options), ("keys_per_s", ctypes.c_double), ("status", ctypes.c_int32),
+                ("pad", ctypes.c_uint32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("tries", ctypes.c_uint64), ("elapsed_s", ctypes.c_double)]
+
+
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(Stats))
+
+
+class SearchConfig(ctypes.Structure):
+    _fields_ = [("pattern", ctypes.c_void_p), ("seed_source", ctypes.c_int32), ("bench", ctypes.c_uint32),
+                ("time_limit_s", ctypes.c_double), ("progress_interval_s", ctypes.c_double),
+                ("target_launch_s", ctypes.c_double), ("cancel", ctypes.POINTER(ctypes.c_int32)),
+                ("key", ctypes.c_char_p)]
+
+
+class Outcome(ctypes.Structure):
+    _fields_ = [("found", ctypes.c_int32), ("pad", ctypes.c_uint32), ("stats", Stats), ("launches", ctypes.c_uint64),
+                ("index", ctypes.c_uint64), ("key", ctypes.c_uint8 * 32), ("seed", ctypes.c_uint8 * 32),
+                ("pk", ctypes.c_uint8 * 32)]
 
 
 @dataclass(frozen=True)
@@ -152,6 +178,13 @@ def lib() -> ctypes.CDLL:
     L.mk_server_now.argtypes = [P]
     L.mk_server_now.restype = ctypes.c_double
     L.mk_server_close.argtypes = [P]
+    L.mk_search.argtypes = [P, ctypes.POINTER(SearchConfig), PROGRESS_FN, P, ctypes.POINTER(Outcome)]
+    L.mk_openssh_public.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t,
+                                    ctypes.POINTER(ctypes.c_size_t)]
+    L.mk_openssh_private.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, ctypes.c_char_p,
+                                     ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    L.mk_fastest.argtypes = [I32, ctypes.POINTER(Options), U32, ctypes.POINTER(TuneResult), ctypes.POINTER(Options)]
+    L.mk_fastest_staged.argtypes = [I32, ctypes.POINTER(Options), ctypes.POINTER(TuneResult), U32, ctypes.POINTER(U32)]
     L.mk_batch_run.argtypes = [P, ctypes.POINTER(SearchSpec), U32, ctypes.POINTER(HitRec), U32,
                                ctypes.POINTER(U32), ctypes.POINTER(U64)]
     _lib = L
@@ -167,6 +200,38 @@ def last_error() -> str:
 def _check(status: int) -> None:
     if status != 0:
         raise MeikeyError(status, last_error())
+
+
+def openssh_public(pk: bytes, comment: str = "") -> str:
+    """authorized_keys line of a public key (VanityKey::public_openssh)."""
+    n = ctypes.c_size_t(0)
+    c = comment.encode()
+    _check(lib().mk_openssh_public(pk, c, None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _check(lib().mk_openssh_public(pk, c, buf, len(buf), ctypes.byref(n)))
+    return buf.value.decode()
+
+
+def openssh_private(seed: bytes, pk: bytes, comment: str = "", checkint: Optional[int] = None) -> str:
+    """Unencrypted openssh-key-v1 private key text (private_openssh)."""
+    if checkint is None:
+        checkint = int.from_bytes(os.urandom(4), "little")
+    n = ctypes.c_size_t(0)
+    c = comment.encode()
+    _check(lib().mk_openssh_private(seed, pk, c, checkint, None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _check(lib().mk_openssh_private(seed, pk, c, checkint, buf, len(buf), ctypes.byref(n)))
+    return buf.value.decode()
+
+
+def fastest_staged(ordinal: int = 0):
+    """The staged autotuner: (best Options, [(width, batch, blocks, keys/s, status), ...])."""
+    best = Options()
+    res = (TuneResult * 64)()
+    n = ctypes.c_uint32(0)
+    _check(lib().mk_fastest_staged(ordinal, ctypes.byref(best), res, 64, ctypes.byref(n)))
+    rows = [(r.options.width, r.options.batch, r.options.blocks, r.keys_per_s, r.status) for r in res[:n.value]]
+    return best, rows
 
 
 def device_count() -> int:
@@ -266,7 +331,7 @@ class Launch:
 
     def run(self, base: int, iters: int, cap: int = 1 << 20) -> List[Hit]:
         """Engine::launch: exact hits of [base, base + keys_per_iter * iters), ascending."""
-        recs = (HitRec * max(cap, 1))()
+        recs = self.gpu._hit_buffer(cap)
         n = ctypes.c_uint32(0)
         _check(lib().mk_launch_run(self._h, base, iters, recs, cap, ctypes.byref(n)))
         return _hits(recs, min(n.value, cap))
@@ -291,6 +356,15 @@ class Gpu:
         h = ctypes.c_void_p()
         _check(lib().mk_open(ordinal, ctypes.byref(self.options), ctypes.byref(h)))
         self._h = h
+        self._hits = None
+
+    def _hit_buffer(self, cap: int):
+        """A hit-record array of at least `cap` entries, allocated once per device handle
+        (ctypes zero-fills new arrays, which would cost milliseconds per launch)."""
+        cap = max(cap, 1)
+        if self._hits is None or len(self._hits) < cap:
+            self._hits = (HitRec * cap)()
+        return self._hits
 
     def info(self) -> Info:
         out = Info()
@@ -329,6 +403,21 @@ class Gpu:
         _check(lib().mk_derive_seeds(self._h, seeds, n, out))
         return out.raw[: 32 * n]
 
+    def search(self, pattern: Optional["Pattern"], seed_source: int = SEED_CHACHA20, bench: bool = False,
+               time_limit: float = 0.0, progress=None, progress_interval: float = 5.0, target_launch: float = 0.2,
+               cancel=None, key: Optional[bytes] = None) -> Outcome:
+        """Searcher::run: search until an exact hit (or, with bench, until time_limit).
+        progress(tries, elapsed) is called every progress_interval seconds; cancel is an
+        optional ctypes.c_int32 that stops the search when set."""
+        cfg = SearchConfig(pattern._h if pattern else None, seed_source, int(bench), time_limit,
+                           progress_interval, target_launch,
+                           ctypes.pointer(cancel) if cancel is not None else None, key)
+        cb = PROGRESS_FN((lambda user, st: progress(st.contents.tries, st.contents.elapsed_s)) if progress
+                         else (lambda user, st: None))
+        out = Outcome()
+        _check(lib().mk_search(self._h, ctypes.byref(cfg), cb, None, ctypes.byref(out)))
+        return out
+
     def last_timing(self):
         """(search kernel ms, confirm kernel ms) of the last launch, from HIP events."""
         a, b = ctypes.c_float(0), ctypes.c_float(0)
@@ -348,7 +437,7 @@ class Gpu:
         keep = []
         for i, s in enumerate(specs):
             _fill_spec(arr[i], s, keep)
-        recs = (HitRec * max(cap, 1))()
+        recs = self._hit_buffer(cap)
         nh = ctypes.c_uint32(0)
         tries = (ctypes.c_uint64 * max(n, 1))()
         _check(lib().mk_batch_run(self._h, arr, n, recs, cap, ctypes.byref(nh), tries))
@@ -389,9 +478,9 @@ class Server:
     """Continuous batching of search requests on one Gpu (mk_server_* of the C ABI)."""
 
     def __init__(self, gpu: Gpu, max_slots: int = 1024, use_graph: bool = True, dfa_budget: int = 64 << 20,
-                 step_keys: int = 1 << 26):
+                 step_keys: int = 1 << 26, active_per_step: int = 64):
         self.gpu = gpu
-        opt = ServerOptions(max_slots, int(use_graph), dfa_budget, step_keys)
+        opt = ServerOptions(max_slots, int(use_graph), dfa_budget, step_keys, active_per_step, 0)
         h = ctypes.c_void_p()
         _check(lib().mk_server_open(gpu._h, ctypes.byref(opt), ctypes.byref(h)))
         self._h = h

@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # gfx950 only; packed-FP32 instructions stay off (the kernels are integer code anyway).
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-HIP_SOURCES = [os.path.join(CSRC, "mk_kernels.hip"), os.path.join(CSRC, "mk_host.cpp"), os.path.join(CSRC, "mk_pattern.cpp"), os.path.join(CSRC, "mk_serve.cpp")]
+HIP_SOURCES = [os.path.join(CSRC, "mk_kernels.hip"), os.path.join(CSRC, "mk_host.cpp"), os.path.join(CSRC, "mk_pattern.cpp"), os.path.join(CSRC, "mk_serve.cpp"), os.path.join(CSRC, "mk_driver.cpp")]
 
 
 def _run(cmd: list[str]) -> None:

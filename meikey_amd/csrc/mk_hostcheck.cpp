@@ -15,6 +15,7 @@
 #include "mk_match.h"
 #include "mk_point.h"
 #include "mk_tile.h"
+#include "mk_search.h"
 
 #define MK_MATCHER_FAIL(st, ...) (fprintf(stderr, __VA_ARGS__), fputc('\n', stderr), (st))
 #include "mk_matcher_host.h"
@@ -137,6 +138,51 @@ int hc_match(const u32 pk[8], const u8* needle, u32 len, u32 fold, u32 mode, u32
     mt.mask = m.mask;
     mt.dfa = m.dfa.empty() ? nullptr : m.dfa.data();
     return mk_pk_match(pk, mt, sign_unknown != 0) ? 1 : 0;
+}
+
+// search_with over a scripted engine and clock (Python callbacks), as the reference tests
+// its control loop with a mock Engine.
+typedef int (*hc_launch_fn)(void* user, u64 base, u32 iters, int* found, u64* index);
+typedef double (*hc_clock_fn)(void* user);
+typedef void (*hc_report_fn)(void* user, u64 tries, double elapsed);
+typedef int (*hc_cancel_fn)(void* user);
+
+struct hc_engine {
+    hc_launch_fn fn;
+    void* user;
+    u64 kpi;
+    u64 keys_per_iter() const { return kpi; }
+    int launch(u64 base, u32 iters, int& found, u64& index) { return fn(user, base, iters, &found, &index); }
+};
+struct hc_clock {
+    hc_clock_fn fn;
+    void* user;
+    double now() { return fn(user); }
+};
+struct hc_progress {
+    hc_report_fn fn;
+    void* user;
+    void report(u64 t, double e) { fn(user, t, e); }
+};
+struct hc_cancel {
+    hc_cancel_fn fn;
+    void* user;
+    bool cancelled() { return fn(user) != 0; }
+};
+
+int hc_search_with(hc_launch_fn launch, hc_clock_fn clock, hc_report_fn report, hc_cancel_fn cancel, void* user,
+                   u64 keys_per_iter, int bench, double target, double time_limit, double interval, u64* out) {
+    hc_engine e{launch, user, keys_per_iter};
+    hc_clock c{clock, user};
+    hc_progress p{report, user};
+    hc_cancel k{cancel, user};
+    mk_search_result r = mk_search_with(e, c, p, k, bench != 0, target, time_limit, interval);
+    out[0] = (u64)r.status;
+    out[1] = (u64)r.found;
+    out[2] = r.index;
+    out[3] = r.tries;
+    out[4] = r.launches;
+    return r.status;
 }
 
 }  // extern "C"

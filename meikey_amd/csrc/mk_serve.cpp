@@ -9,7 +9,9 @@
 // once it has a candidate, and the confirm kernel turns candidates into exact hits.
 // After the step the host reads the counters, retires every slot with an exact hit
 // (its lowest-index hit of the step is the result) or with its candidate budget spent,
-// and admits waiting requests. The step -- slot-list upload, counter reset, search,
+// and admits waiting requests. Each step serves at most active_per_step resident requests,
+// those with the fewest candidates searched so far (least attained service, which favours
+// the short requests of a heavy-tailed mix without knowing their sizes). The step -- slot-list upload, counter reset, search,
 // confirm, counter and next-index download -- is captured once as a hipGraph and
 // replayed; its kernels clear their own counters (no memset nodes).
 #include <hip/hip_runtime.h>
@@ -45,7 +47,7 @@ struct pending {
 struct slot_state {
     bool used = false;
     u64 request = 0;
-    u64 first = 0, count = 0;
+    u64 first = 0, count = 0, tries = 0;
     size_t dfa_off = 0, dfa_n = 0;  // u16 units in the pool
     double submit = 0, admit = 0;
 };
@@ -225,6 +227,7 @@ void mk_server_default_options(mk_server_options* out) {
     out->use_graph = 1;
     out->dfa_budget = 64ull << 20;
     out->step_keys = 1ull << 26;
+    out->active_per_step = 64;
 }
 
 mk_status mk_server_open(mk_gpu* g, const mk_server_options* opt_in, mk_server** out) {
@@ -306,9 +309,20 @@ mk_status mk_server_step(mk_server* s, uint32_t* active_out, uint32_t* queued_ou
     MK_HIPS(hipSetDevice(g->device));
     mk_status st = admit(s);
     if (st != MK_OK) return st;
-    u32 n = 0;
+    // least attained service: the resident requests with the fewest candidates so far
+    std::vector<std::pair<u64, u32>> order;
     for (u32 i = 0; i < s->opt.max_slots; i++)
-        if (s->slots[i].used) s->h_serve->slots[n++] = i;
+        if (s->slots[i].used) order.emplace_back(s->slots[i].tries, i);
+    const size_t cap = s->opt.active_per_step ? s->opt.active_per_step : order.size();
+    if (order.size() > cap) {
+        std::nth_element(order.begin(), order.begin() + cap, order.end());
+        order.resize(cap);
+    }
+    std::sort(order.begin(), order.end(), [](const std::pair<u64, u32>& a, const std::pair<u64, u32>& b) {
+        return a.second < b.second;
+    });
+    u32 n = 0;
+    for (const auto& o : order) s->h_serve->slots[n++] = o.second;
     s->h_serve->n_active = n;
     if (active_out) *active_out = n;
     if (queued_out) *queued_out = (u32)s->queue.size();
@@ -336,6 +350,7 @@ mk_status mk_server_step(mk_server* s, uint32_t* active_out, uint32_t* queued_ou
     for (u32 i = 0; i < n; i++) {
         const u32 slot = s->h_serve->slots[i];
         const u64 tries = s->h_next[slot] - s->slots[slot].first;
+        s->slots[slot].tries = tries;
         auto it = best.find(slot);
         if (it != best.end()) {
             finish(s, slot, 1, &it->second, tries);

@@ -220,37 +220,9 @@ def test_regex_search_vs_oracle(gpu, oracle, text, cs):
         assert rx.search(T.hay(h.pk)) is not None
 
 
-def config4_specs(count, rng_seed=4):
-    """Config 4: 64 searches with master keys from PRNG seeds 1000..1063, kinds 25% each:
-    case-insensitive substring (3-7), case-sensitive substring, suffix (3-6), regex with
-    at most 64 DFA states (random anchored / unanchored)."""
-    rng = random.Random(rng_seed)
-    alpha = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
-    pats = []
-    for i in range(64):
-        kind = i % 4
-        if kind == 0:
-            p = M.Pattern("".join(rng.choice(alpha) for _ in range(rng.randint(3, 7))), False)
-        elif kind == 1:
-            p = M.Pattern("".join(rng.choice(alpha) for _ in range(rng.randint(3, 7))), True)
-        elif kind == 2:
-            p = M.Pattern("".join(rng.choice(alpha) for _ in range(rng.randint(3, 6))), rng.random() < 0.5,
-                          M.MODE_SUFFIX)
-        else:
-            while True:
-                body = "".join(rng.choice(["[A-Z]", "[0-9]", "[a-f]", rng.choice(alpha[:62]), "."])
-                               for _ in range(rng.randint(2, 4)))
-                text = rng.choice(["^", "", ""]) + body + rng.choice(["$", "", ""])
-                try:
-                    p = M.Pattern(text, True, M.MODE_REGEX)
-                except M.PatternError:
-                    continue
-                # at most 64 DFA states, and rare enough to be a search (>= 2^10 expected tries)
-                if p.dfa_states <= 64 and p.probability <= 2.0 ** -10:
-                    break
-        pats.append(p)
-    keys = [random.Random(1000 + i).randbytes(32) for i in range(64)]
-    return [p.spec(k, 0, count) for p, k in zip(pats, keys)], pats, keys
+def config4_specs(count):
+    from meikey_amd import workloads as W
+    return W.batched(count)
 
 
 def test_config4_batched_equals_single_and_oracle(gpu, oracle):
