@@ -224,6 +224,16 @@ def openssh_private(seed: bytes, pk: bytes, comment: str = "", checkint: Optiona
     return buf.value.decode()
 
 
+def fastest(ordinal: int, candidates):
+    """Measure each (width, batch, blocks) candidate: (best Options, rows as fastest_staged)."""
+    cands = (Options * len(candidates))(*[Options(w, b, LIMBS_R25, k) for w, b, k in candidates])
+    res = (TuneResult * len(candidates))()
+    best = Options()
+    _check(lib().mk_fastest(ordinal, cands, len(candidates), res, ctypes.byref(best)))
+    rows = [(r.options.width, r.options.batch, r.options.blocks, r.keys_per_s, r.status) for r in res]
+    return best, rows
+
+
 def fastest_staged(ordinal: int = 0):
     """The staged autotuner: (best Options, [(width, batch, blocks, keys/s, status), ...])."""
     best = Options()

@@ -351,3 +351,57 @@ def test_server_budget_and_cancel(gpu):
         assert done[a].status in (0, 1)
         if done[a].status == 0:
             assert done[a].tries >= 1 << 21
+
+
+def test_searcher_exact_hit_and_openssh(gpu, oracle, tmp_path):
+    """Searcher::run (search.rs:34-40, gpu.rs:999-1012): the reported key is a real hit of
+    the pattern, derived from the master key at the reported index; its OpenSSH files
+    carry the same key."""
+    from tests.test_openssh import parse_private
+    key = random.Random(60).randbytes(32)
+    for text, cs, mode in [("Zq", True, M.MODE_SUFFIX), ("k9w", False, M.MODE_CONTAINS), ("^[A-D][0-9]", True, M.MODE_REGEX)]:
+        say(f"searcher {text!r}")
+        pat = M.Pattern(text, cs, mode)
+        seen = []
+        out = gpu.search(pat, M.SEED_CHACHA20, key=key, progress=lambda t, e: seen.append(t), progress_interval=1e-3)
+        assert out.found == 1 and bytes(out.key) == key
+        seed = oracle.seed(key, M.SEED_CHACHA20, out.index)
+        assert bytes(out.seed) == seed
+        pk = oracle.derive(key, M.SEED_CHACHA20, out.index, 1)
+        assert bytes(out.pk) == pk
+        assert pat.is_match(T.public_b64(pk))
+        assert out.stats.tries >= out.index + 1 and out.launches >= 1
+        assert seen == sorted(seen)
+        import base64
+        line = M.openssh_public(pk, "vanity")
+        assert line.split() == ["ssh-ed25519", base64.b64encode(b"\0\0\0\x0bssh-ed25519\0\0\0\x20" + pk).decode(),
+                                "vanity"]
+        _, _, _, _, pub, priv = parse_private(M.openssh_private(seed, pk, "vanity", 7))
+        assert pub.endswith(pk) and seed + pk in priv
+
+
+def test_searcher_bench_runs_to_the_time_limit(gpu_small):
+    pat = M.Pattern("A", True, M.MODE_CONTAINS)
+    out = gpu_small.search(pat, M.SEED_COUNTER, bench=True, time_limit=0.5, target_launch=0.05)
+    assert out.found == 0
+    assert out.launches >= 3 and out.stats.elapsed_s >= 0.5
+    assert out.stats.tries % gpu_small.keys_per_iter == 0
+
+
+def test_searcher_cancel(gpu_small):
+    import ctypes
+    flag = ctypes.c_int32(1)
+    out = gpu_small.search(M.Pattern("AAAAAAAAAA", True, M.MODE_CONTAINS), M.SEED_COUNTER, cancel=flag)
+    assert out.found == 0 and out.launches == 1
+
+
+def test_autotuner_measures_and_picks_the_fastest():
+    """fastest (gpu.rs:1014-1060): every setting opened, self-checked and measured; an
+    invalid one is rejected with its status; the best is the measured maximum."""
+    say("autotune W=12/14")
+    best, rows = M.fastest(0, [(12, 8, 1), (14, 16, 2), (40, 8, 1)])
+    assert rows[2][4] == 3 and rows[2][3] == 0  # out of range
+    ok = [r for r in rows if r[4] == 0]
+    assert len(ok) == 2 and all(r[3] > 1e6 for r in ok)
+    top = max(ok, key=lambda r: r[3])
+    assert (best.width, best.batch, best.blocks) == top[:3]
