@@ -48,6 +48,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-search batched leg")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="serving run length (0 = skip)")
+    ap.add_argument("--no-low", action="store_true", help="skip the low-security (counter seed) leg")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -55,8 +56,8 @@ def main():
         copy_calibration(args.copy_bytes)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each search shards by index "
-              "range across GPUs with no collectives; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on one GPU; a "
+              "search would shard by disjoint index ranges across GPUs, with no collectives)", file=sys.stderr)
         sys.exit(2)
 
     import meikey_amd as M
@@ -88,8 +89,14 @@ def main():
     elapsed = time.perf_counter() - t0
     rate = keys / elapsed
     avg_ms = sum(kernel_ms) / len(kernel_ms)
-    mults = field_mults_per_key(args.width, args.batch)
+    # Algorithmic work per key is the canonical W=16, K=256 figure (106.04 field
+    # multiplications x 64 products = 6,787 32x32-bit products), whatever width and batch
+    # the run uses, so the achieved rate is proportional to keys/s; the run's own count
+    # (fewer multiplications at larger widths) is reported beside it.
+    mults = field_mults_per_key(16, 256)
+    run_mults = field_mults_per_key(args.width, args.batch)
     achieved = STEP_KEYS * mults * 64 / (avg_ms * 1e-3)
+    achieved_run = STEP_KEYS * run_mults * 64 / (avg_ms * 1e-3)
     traffic = measured_traffic(args.width, args.batch, args.blocks)
     out = {
         "metric": METRIC, "value": rate, "unit": "keys/s", "n_gpus": 1, "steps": args.steps,
@@ -103,16 +110,22 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_MUL / 1e12, "unit": "TOP/s",
                      "frac": achieved / PEAK_MUL, "traffic": traffic,
                      "kernel": "mk_search_kernel", "kernel_ms": avg_ms,
-                     "algorithmic": f"{mults:.2f} field mults/key x 64 32x32-bit products"},
+                     "algorithmic": f"{mults:.2f} field mults/key (W=16, K=256) x 64 32x32-bit products",
+                     "achieved_run_mults": achieved_run / 1e12,
+                     "run_mults_per_key": run_mults},
     }
     if not args.no_batch:
         out["batched"] = batched_leg(gpu, rate)
         out["batched"]["kernel_vs_single_kernel"] = out["batched"]["kernel_keys_per_s"] / (STEP_KEYS / (avg_ms * 1e-3))
+    low = None
+    if not args.no_low:
+        out["low_security"], low = low_security_leg(gpu, args)
+        out["low_security"]["vs_secure"] = out["low_security"]["value"] / rate
     served = []
     if args.serve_seconds > 0:
         out["serving"], served = serving_leg(gpu, args.serve_seconds)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(gpu, key, needle, served)
+        out["cpu_baseline"] = cpu_baseline(gpu, key, needle, served, low)
     gpu.close()
     print(json.dumps(out), flush=True)
 
@@ -125,7 +138,7 @@ def batched_leg(gpu, single_rate):
     warm = W.batched(1 << 20)[0]
     gpu.batch_run(warm)
     t0 = time.perf_counter()
-    hits, tries = gpu.batch_run(specs)
+    hits, tries = gpu.batch_run(specs, as_array=True)
     dt = time.perf_counter() - t0
     batched_ms = gpu.last_timing()[0]
     total = sum(tries)
@@ -133,14 +146,12 @@ def batched_leg(gpu, single_rate):
     single_hits = []
     single_ms = 0.0
     for s in specs:
-        h, _ = gpu.batch_run([s])
+        h, _ = gpu.batch_run([s], as_array=True)
         single_ms += gpu.last_timing()[0]
-        single_hits.append([x.index for x in h])
+        single_hits.append(h["index"])
     dt1 = time.perf_counter() - t0
-    by = [[] for _ in specs]
-    for h in hits:
-        by[h.search].append(h.index)
-    mismatched = sum(1 for a, b in zip(by, single_hits) if a != b)
+    by = [hits["index"][hits["search"] == i] for i in range(len(specs))]
+    mismatched = sum(1 for a, b in zip(by, single_hits) if a.tolist() != b.tolist())
     rate = total / dt
     print(f"# batched 64 searches: {rate / 1e9:.3f} G keys/s, one per launch {total / dt1 / 1e9:.3f} G keys/s, "
           f"{len(hits)} hits, {mismatched} searches differ", file=sys.stderr, flush=True)
@@ -151,6 +162,42 @@ def batched_leg(gpu, single_rate):
             "one_per_launch_kernel_keys_per_s": total / (single_ms * 1e-3),
             "kinds": "16 substring case-insensitive 3-7, 16 substring case-sensitive 3-7, 16 suffix 3-6, "
                      "16 regex (<= 64 DFA states)"}
+
+
+def low_security_leg(gpu, args):
+    """Low-security mode: counter seeds of the master key of random.Random(3), an 8-char
+    case-insensitive substring, 2^32 candidates in 16 launches. Beside it the same search
+    with a cache-resident comb (W = 6: 42 rows x 33 entries x 128 B = 177 KB, the size
+    class of an LDS-resident table) to show the table-size / field-work trade."""
+    import random
+    import meikey_amd as M
+    from meikey_amd import workloads as W
+    rng = random.Random(3)
+    key = rng.randbytes(32)
+    needle = "".join(rng.choice(W.ALPHABET) for _ in range(8)).lower().encode()
+
+    def run(g, launches, step):
+        spec = {"key": key, "seed_source": M.SEED_COUNTER, "needle": needle, "fold": True, "mode": M.MODE_CONTAINS}
+        g.batch_run([dict(spec, first=1 << 40, count=step)])
+        hits, ms, t0 = [], 0.0, time.perf_counter()
+        for i in range(launches):
+            h, _ = g.batch_run([dict(spec, first=i * step, count=step)])
+            hits += h
+            ms += g.last_timing()[0]
+        return launches * step / (time.perf_counter() - t0), launches * step / (ms * 1e-3), hits
+
+    rate, kernel_rate, hits = run(gpu, 16, STEP_KEYS)
+    small = M.Gpu(0, width=6, batch=args.batch, blocks=args.blocks)
+    small_bytes = small.info().table_bytes
+    small_rate, small_kernel, _ = run(small, 2, 1 << 26)
+    small.close()
+    print(f"# low-security: {rate / 1e9:.3f} G keys/s (W={args.width}), {small_rate / 1e9:.3f} G keys/s (W=6 "
+          f"cache-resident table), {len(hits)} hits", file=sys.stderr, flush=True)
+    return ({"value": rate, "unit": "keys/s", "candidates": 16 * STEP_KEYS, "needle": needle.decode(),
+             "kernel_keys_per_s": kernel_rate, "hits": len(hits), "vs_secure": None,
+             "w6_table": {"keys_per_s": small_rate, "kernel_keys_per_s": small_kernel, "table_bytes": small_bytes,
+                          "candidates": 2 << 26}},
+            (key, needle))
 
 
 def percentile(xs, q):
@@ -240,10 +287,11 @@ def copy_calibration(nbytes: int) -> None:
     print(json.dumps({"copy_bytes": nbytes, "dispatches": 3}), flush=True)
 
 
-def cpu_baseline(gpu, key, needle, served=()):
+def cpu_baseline(gpu, key, needle, served=(), low=None):
     """The CPU oracle (a scalar C port of the reference's per-key algorithm) on this
     host's cores over a bounded sample of the same workload; the GPU's public keys of
-    the same sample, and every key the server returned, are checked against it."""
+    the same sample, every key the server returned, and (low-security leg) the keys of
+    every 2^20-th counter-seed index are checked against it."""
     from oracle import oracle as O
 
     cores = min(os.cpu_count() or 1, 16)
@@ -262,8 +310,16 @@ def cpu_baseline(gpu, key, needle, served=()):
         pk = O.derive(req.key, 0, r.index, 1)
         if pk != r.pk or O.seed(req.key, 0, r.index) != r.seed or not req.pattern.is_match(O.public_b64(pk)):
             bad_served += 1
+    low_checked = low_wrong = 0
+    if low is not None:
+        lkey, _ = low
+        for i in range(0, 1 << 32, 1 << 20):
+            low_checked += 1
+            if gpu.derive(lkey, 1, i, 1) != O.derive(lkey, 1, i, 1):
+                low_wrong += 1
     return {"value": chunks * per / dt, "unit": "keys/s", "cores": cores, "kind": "port",
             "served_keys_checked": len(served), "served_keys_wrong": bad_served,
+            "low_security_keys_checked": low_checked, "low_security_keys_wrong": low_wrong,
             "sample": f"{chunks} x {per} consecutive candidates spread over the 2^32 range, derive + suffix match "
                       f"with the C oracle; GPU public keys of the same 2^20 candidates compared: "
                       f"{'all equal' if mismatches == 0 else f'{mismatches} chunks differ'}",

@@ -264,6 +264,24 @@ def _fill_spec(dst: "SearchSpec", s: dict, keep: list) -> None:
     dst.count = s.get("count", 0)
 
 
+HIT_DTYPE = None
+
+
+def hit_dtype():
+    """numpy dtype of mk_hit_rec (index u64, search u32, pad u32, seed[32], pk[32])."""
+    global HIT_DTYPE
+    if HIT_DTYPE is None:
+        import numpy as np
+        HIT_DTYPE = np.dtype([("index", "<u8"), ("search", "<u4"), ("pad", "<u4"), ("seed", "u1", 32), ("pk", "u1", 32)])
+        assert HIT_DTYPE.itemsize == ctypes.sizeof(HitRec)
+    return HIT_DTYPE
+
+
+def _hits_array(recs, n: int):
+    import numpy as np
+    return np.frombuffer(recs, dtype=hit_dtype(), count=n).copy()
+
+
 def _hits(recs, n: int) -> List[Hit]:
     return [Hit(recs[i].index, recs[i].search, bytes(recs[i].seed), bytes(recs[i].pk)) for i in range(n)]
 
@@ -439,9 +457,10 @@ class Gpu:
         _check(lib().mk_table_read(self._h, row, k, count, buf))
         return list(buf)
 
-    def batch_run(self, specs: Sequence[dict], cap: int = 1 << 20):
+    def batch_run(self, specs: Sequence[dict], cap: int = 1 << 20, as_array: bool = False):
         """Run many searches in one launch. Each spec: key, seed_source, mode, fold, needle,
-        first, count, stop_on_hit (optional). Returns (hits, tries per search)."""
+        first, count, stop_on_hit (optional). Returns (hits, tries per search); with
+        as_array the hits are one numpy structured array (fields of hit_dtype())."""
         n = len(specs)
         arr = (SearchSpec * max(n, 1))()
         keep = []
@@ -451,7 +470,8 @@ class Gpu:
         nh = ctypes.c_uint32(0)
         tries = (ctypes.c_uint64 * max(n, 1))()
         _check(lib().mk_batch_run(self._h, arr, n, recs, cap, ctypes.byref(nh), tries))
-        return _hits(recs, min(nh.value, cap)), list(tries)[:n]
+        got = min(nh.value, cap)
+        return (_hits_array(recs, got) if as_array else _hits(recs, got)), list(tries)[:n]
 
     def close(self) -> None:
         if getattr(self, "_h", None):

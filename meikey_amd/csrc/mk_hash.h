@@ -56,58 +56,65 @@ MK_DEV void mk_counter_seed(const u32 key[8], u64 index, u32 seed[8]) {
 
 MK_DEV u32 mk_bswap32(u32 x) { return (x >> 24) | ((x >> 8) & 0xff00u) | ((x << 8) & 0xff0000u) | (x << 24); }
 
-// 64-bit values as (hi, lo) 32-bit halves so that every rotate is two funnel shifts
-// (v_alignbit_b32) and three-input XOR / majority are one v_bitop3_b32 each on gfx950.
-struct mk_u64p {
-    u32 lo, hi;
-};
-
+// SHA-512 word operations on the two 32-bit halves of a u64: every rotate is two funnel
+// shifts (v_alignbit_b32), three-input XOR, choose and majority are one v_bitop3_b32 per
+// half on gfx950, and 64-bit additions stay single v_lshl_add_u64. The halves are taken
+// and rejoined through a <2 x u32> bit cast, which the compiler keeps as plain register
+// pairs (building the u64 with shift-and-or instead lets it split the later 64-bit sums
+// into 32-bit pieces with extra moves).
 #if defined(__HIP_DEVICE_COMPILE__)
+typedef u32 mk_u32x2 __attribute__((ext_vector_type(2)));
+MK_DEV u32 mk_lo(u64 x) { return __builtin_bit_cast(mk_u32x2, x).x; }
+MK_DEV u32 mk_hi(u64 x) { return __builtin_bit_cast(mk_u32x2, x).y; }
+MK_DEV u64 mk_join(u32 lo, u32 hi) {
+    mk_u32x2 v;
+    v.x = lo;
+    v.y = hi;
+    return __builtin_bit_cast(u64, v);
+}
 MK_DEV u32 mk_funnel(u32 hi, u32 lo, u32 n) { return __builtin_amdgcn_alignbit(hi, lo, n); }
+// v_bitop3_b32 truth tables index the inputs as a = 0xF0, b = 0xCC, c = 0xAA:
+// a ^ b ^ c = 0x96, maj(a, b, c) = 0xE8, a ? b : c = 0xCA.
 MK_DEV u32 mk_xor3(u32 a, u32 b, u32 c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96); }
 MK_DEV u32 mk_maj(u32 a, u32 b, u32 c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0xE8); }
+MK_DEV u32 mk_choose(u32 a, u32 b, u32 c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0xCA); }
 #else
+MK_DEV u32 mk_lo(u64 x) { return (u32)x; }
+MK_DEV u32 mk_hi(u64 x) { return (u32)(x >> 32); }
+MK_DEV u64 mk_join(u32 lo, u32 hi) { return (u64)hi << 32 | lo; }
 // (hi:lo) >> n, low 32 bits, for n in 0..31
 MK_DEV u32 mk_funnel(u32 hi, u32 lo, u32 n) { return (u32)((((u64)hi << 32) | lo) >> (n & 31)); }
 MK_DEV u32 mk_xor3(u32 a, u32 b, u32 c) { return a ^ b ^ c; }
 MK_DEV u32 mk_maj(u32 a, u32 b, u32 c) { return (a & b) | (a & c) | (b & c); }
+MK_DEV u32 mk_choose(u32 a, u32 b, u32 c) { return (a & b) | (~a & c); }
 #endif
 
-// rotate right by a constant n in 1..63
-MK_DEV mk_u64p mk_rotr(mk_u64p x, int n) {
-    mk_u64p r;
-    if (n < 32) {
-        r.lo = mk_funnel(x.hi, x.lo, (u32)n);
-        r.hi = mk_funnel(x.lo, x.hi, (u32)n);
-    } else {
-        r.lo = mk_funnel(x.lo, x.hi, (u32)(n - 32));
-        r.hi = mk_funnel(x.hi, x.lo, (u32)(n - 32));
-    }
-    return r;
+// Halves of x rotated right by a constant n in 1..63.
+MK_DEV u32 mk_rotr_lo(u64 x, int n) {
+    return n < 32 ? mk_funnel(mk_hi(x), mk_lo(x), (u32)n) : mk_funnel(mk_lo(x), mk_hi(x), (u32)(n - 32));
+}
+MK_DEV u32 mk_rotr_hi(u64 x, int n) {
+    return n < 32 ? mk_funnel(mk_lo(x), mk_hi(x), (u32)n) : mk_funnel(mk_hi(x), mk_lo(x), (u32)(n - 32));
 }
 
-// shift right by a constant n in 1..31
-MK_DEV mk_u64p mk_shr(mk_u64p x, int n) {
-    mk_u64p r;
-    r.lo = mk_funnel(x.hi, x.lo, (u32)n);
-    r.hi = x.hi >> n;
-    return r;
+// rotr(x, a) ^ rotr(x, b) ^ rotr(x, c): the round functions Sigma0 / Sigma1
+MK_DEV u64 mk_rot3(u64 x, int a, int b, int c) {
+    return mk_join(mk_xor3(mk_rotr_lo(x, a), mk_rotr_lo(x, b), mk_rotr_lo(x, c)),
+                   mk_xor3(mk_rotr_hi(x, a), mk_rotr_hi(x, b), mk_rotr_hi(x, c)));
 }
 
-MK_DEV mk_u64p mk_x3(mk_u64p a, mk_u64p b, mk_u64p c) {
-    mk_u64p r;
-    r.lo = mk_xor3(a.lo, b.lo, c.lo);
-    r.hi = mk_xor3(a.hi, b.hi, c.hi);
-    return r;
+// rotr(x, a) ^ rotr(x, b) ^ (x >> c), c in 1..31: the schedule functions sigma0 / sigma1
+MK_DEV u64 mk_rot2_shr(u64 x, int a, int b, int c) {
+    return mk_join(mk_xor3(mk_rotr_lo(x, a), mk_rotr_lo(x, b), mk_funnel(mk_hi(x), mk_lo(x), (u32)c)),
+                   mk_xor3(mk_rotr_hi(x, a), mk_rotr_hi(x, b), mk_hi(x) >> c));
 }
 
-MK_DEV u64 mk_v(mk_u64p x) { return (u64)x.hi << 32 | x.lo; }
+MK_DEV u64 mk_ch64(u64 e, u64 f, u64 g) {
+    return mk_join(mk_choose(mk_lo(e), mk_lo(f), mk_lo(g)), mk_choose(mk_hi(e), mk_hi(f), mk_hi(g)));
+}
 
-MK_DEV mk_u64p mk_p(u64 v) {
-    mk_u64p r;
-    r.lo = (u32)v;
-    r.hi = (u32)(v >> 32);
-    return r;
+MK_DEV u64 mk_maj64(u64 a, u64 b, u64 c) {
+    return mk_join(mk_maj(mk_lo(a), mk_lo(b), mk_lo(c)), mk_maj(mk_hi(a), mk_hi(b), mk_hi(c)));
 }
 
 // FIPS 180-4 round constants.
@@ -139,41 +146,48 @@ MK_CONST u64 MK_K512[80] = MK_SHA512_K;
 // variables rotate through the call sites instead of being moved, and w[] is indexed
 // only by constants, so the 80 rounds become straight-line code with no register
 // indexing.
-template <int T>
+// Mid = true: W16 and W17 are already in w[0] and w[1] (mk_sha_mid).
+template <int T, bool Mid = false>
 MK_DEV void mk_sha_round(u64 w[16], u64 a, u64 b, u64 c, u64& d, u64 e, u64 f, u64 g, u64& h) {
     u64 wt;
-    if (T < 16) {
-        wt = w[T];
+    if (T < 16 || (Mid && T < 18)) {
+        wt = w[T & 15];
     } else {
-        const mk_u64p w15 = mk_p(w[(T + 1) & 15]), w2 = mk_p(w[(T + 14) & 15]);
-        const u64 s0 = mk_v(mk_x3(mk_rotr(w15, 1), mk_rotr(w15, 8), mk_shr(w15, 7)));
-        const u64 s1 = mk_v(mk_x3(mk_rotr(w2, 19), mk_rotr(w2, 61), mk_shr(w2, 6)));
-        wt = w[T & 15] + s0 + w[(T + 9) & 15] + s1;
+        const u64 w15 = w[(T + 1) & 15], w2 = w[(T + 14) & 15];
+        wt = w[T & 15] + mk_rot2_shr(w15, 1, 8, 7) + w[(T + 9) & 15] + mk_rot2_shr(w2, 19, 61, 6);
         w[T & 15] = wt;
     }
-    const mk_u64p ep = mk_p(e), ap = mk_p(a), bp = mk_p(b), cp = mk_p(c);
-    const u64 S1 = mk_v(mk_x3(mk_rotr(ep, 14), mk_rotr(ep, 18), mk_rotr(ep, 41)));
-    const u64 ch = (e & f) ^ (~e & g);
-    const u64 t1 = h + S1 + ch + (MK_K512[T] + wt);
-    const u64 S0 = mk_v(mk_x3(mk_rotr(ap, 28), mk_rotr(ap, 34), mk_rotr(ap, 39)));
-    mk_u64p mj;
-    mj.lo = mk_maj(ap.lo, bp.lo, cp.lo);
-    mj.hi = mk_maj(ap.hi, bp.hi, cp.hi);
-    d += t1;                // the new e
-    h = t1 + S0 + mk_v(mj);  // the new a
+    const u64 t1 = h + mk_rot3(e, 14, 18, 41) + mk_ch64(e, f, g) + (MK_K512[T] + wt);
+    d += t1;                                                  // the new e
+    h = t1 + mk_rot3(a, 28, 34, 39) + mk_maj64(a, b, c);      // the new a
 }
 
 // Eight rounds from T; afterwards the variables are back in their original roles.
-template <int T>
+template <int T, bool Mid = false>
 MK_DEV void mk_sha_8rounds(u64 w[16], u64& a, u64& b, u64& c, u64& d, u64& e, u64& f, u64& g, u64& h) {
-    mk_sha_round<T + 0>(w, a, b, c, d, e, f, g, h);
-    mk_sha_round<T + 1>(w, h, a, b, c, d, e, f, g);
-    mk_sha_round<T + 2>(w, g, h, a, b, c, d, e, f);
-    mk_sha_round<T + 3>(w, f, g, h, a, b, c, d, e);
-    mk_sha_round<T + 4>(w, e, f, g, h, a, b, c, d);
-    mk_sha_round<T + 5>(w, d, e, f, g, h, a, b, c);
-    mk_sha_round<T + 6>(w, c, d, e, f, g, h, a, b);
-    mk_sha_round<T + 7>(w, b, c, d, e, f, g, h, a);
+    mk_sha_round<T + 0, Mid>(w, a, b, c, d, e, f, g, h);
+    mk_sha_round<T + 1, Mid>(w, h, a, b, c, d, e, f, g);
+    mk_sha_round<T + 2, Mid>(w, g, h, a, b, c, d, e, f);
+    mk_sha_round<T + 3, Mid>(w, f, g, h, a, b, c, d, e);
+    mk_sha_round<T + 4, Mid>(w, e, f, g, h, a, b, c, d);
+    mk_sha_round<T + 5, Mid>(w, d, e, f, g, h, a, b, c);
+    mk_sha_round<T + 6, Mid>(w, c, d, e, f, g, h, a, b);
+    mk_sha_round<T + 7, Mid>(w, b, c, d, e, f, g, h, a);
+}
+
+// Big-endian SHA-512 word of two little-endian seed words.
+MK_DEV u64 mk_be_word(u32 lo_word, u32 hi_word) { return (u64)mk_bswap32(lo_word) << 32 | mk_bswap32(hi_word); }
+
+// H0..H3 of the final state as the clamped scalar (8 LE words).
+MK_DEV void mk_sha_finish(u64 a, u64 b, u64 c, u64 d, const u64 iv[8], u32 s_out[8]) {
+    const u64 H[4] = {a + iv[0], b + iv[1], c + iv[2], d + iv[3]};
+    MK_UNROLL
+    for (int i = 0; i < 4; i++) {
+        s_out[2 * i] = mk_bswap32((u32)(H[i] >> 32));
+        s_out[2 * i + 1] = mk_bswap32((u32)H[i]);
+    }
+    s_out[0] &= ~7u;
+    s_out[7] = (s_out[7] & 0x7fffffffu) | 0x40000000u;
 }
 
 // clamp(SHA-512(seed)[..32]) for a 32-byte seed given as 8 LE words; the scalar is
@@ -183,7 +197,7 @@ MK_DEV void mk_secret_scalar(const u32 seed[8], u32 s_out[8]) {
     const u64 iv[8] = MK_SHA512_IV;
     u64 w[16];
     MK_UNROLL
-    for (int i = 0; i < 4; i++) w[i] = (u64)mk_bswap32(seed[2 * i]) << 32 | mk_bswap32(seed[2 * i + 1]);
+    for (int i = 0; i < 4; i++) w[i] = mk_be_word(seed[2 * i], seed[2 * i + 1]);
     w[4] = 0x8000000000000000ull;
     MK_UNROLL
     for (int i = 5; i < 15; i++) w[i] = 0;
@@ -199,14 +213,80 @@ MK_DEV void mk_secret_scalar(const u32 seed[8], u32 s_out[8]) {
     mk_sha_8rounds<56>(w, a, b, c, d, e, f, g, h);
     mk_sha_8rounds<64>(w, a, b, c, d, e, f, g, h);
     mk_sha_8rounds<72>(w, a, b, c, d, e, f, g, h);
-    const u64 H[4] = {a + iv[0], b + iv[1], c + iv[2], d + iv[3]};
+    mk_sha_finish(a, b, c, d, iv, s_out);
+}
+
+// Low-security (counter) seeds share their first 24 bytes, so SHA-512 words W0..W2 are
+// the same for every index of a search and only W3 (bytes 24..31) varies (reference
+// kernel.cu:308-362, ShaMid). The mid-state holds what depends on W0..W2 alone: the
+// working variables after rounds 0-2 with round 3's constant part folded into the two
+// variables it writes (so round 3 costs two additions per key), the schedule words
+// W16 and W17, and W2 for W18.
+struct mk_sha_mid {
+    u64 v[8];   // a..h after round 2; v[0] and v[4] hold round 3's constant parts
+    u64 w2, w16, w17;
+};
+
+MK_DEV void mk_sha_mid_init(const u32 key[8], mk_sha_mid& m) {
+    const u64 iv[8] = MK_SHA512_IV;
+    u64 w[16];
     MK_UNROLL
-    for (int i = 0; i < 4; i++) {
-        s_out[2 * i] = mk_bswap32((u32)(H[i] >> 32));
-        s_out[2 * i + 1] = mk_bswap32((u32)H[i]);
-    }
-    s_out[0] &= ~7u;
-    s_out[7] = (s_out[7] & 0x7fffffffu) | 0x40000000u;
+    for (int i = 0; i < 3; i++) w[i] = mk_be_word(key[2 * i], key[2 * i + 1]);
+    w[3] = 0;  // not read by rounds 0-2
+    w[4] = 0x8000000000000000ull;
+    MK_UNROLL
+    for (int i = 5; i < 15; i++) w[i] = 0;
+    w[15] = 256;
+    u64 a = iv[0], b = iv[1], c = iv[2], d = iv[3], e = iv[4], f = iv[5], g = iv[6], h = iv[7];
+    mk_sha_round<0>(w, a, b, c, d, e, f, g, h);
+    mk_sha_round<1>(w, h, a, b, c, d, e, f, g);
+    mk_sha_round<2>(w, g, h, a, b, c, d, e, f);
+    // round 3 runs as mk_sha_round<3>(w, f, g, h, a, b, c, d, e): it adds t1 = P + W3 to
+    // the variable in slot a and sets slot e to t1 + T2
+    const u64 P = e + mk_rot3(b, 14, 18, 41) + mk_ch64(b, c, d) + MK_K512[3];
+    const u64 T2 = mk_rot3(f, 28, 34, 39) + mk_maj64(f, g, h);
+    m.v[0] = a + P;
+    m.v[1] = b;
+    m.v[2] = c;
+    m.v[3] = d;
+    m.v[4] = P + T2;
+    m.v[5] = f;
+    m.v[6] = g;
+    m.v[7] = h;
+    m.w2 = w[2];
+    m.w16 = w[0] + mk_rot2_shr(w[1], 1, 8, 7) + w[9] + mk_rot2_shr(w[14], 19, 61, 6);
+    m.w17 = w[1] + mk_rot2_shr(w[2], 1, 8, 7) + w[10] + mk_rot2_shr(w[15], 19, 61, 6);
+}
+
+// clamp(SHA-512(seed)[..32]) of a counter seed whose bytes 24..31 are the big-endian
+// word w3, from the search's mid-state. Equal to mk_secret_scalar of the full seed.
+MK_DEV void mk_secret_scalar_mid(const mk_sha_mid& m, u64 w3, u32 s_out[8]) {
+    const u64 iv[8] = MK_SHA512_IV;
+    u64 w[16];
+    // slots 0 and 1 carry W16 and W17, which rounds 16 and 17 then only read
+    w[0] = m.w16;
+    w[1] = m.w17;
+    w[2] = m.w2;
+    w[3] = w3;
+    w[4] = 0x8000000000000000ull;
+    MK_UNROLL
+    for (int i = 5; i < 15; i++) w[i] = 0;
+    w[15] = 256;
+    u64 a = m.v[0] + w3, b = m.v[1], c = m.v[2], d = m.v[3], e = m.v[4] + w3, f = m.v[5], g = m.v[6], h = m.v[7];
+    mk_sha_round<4, true>(w, e, f, g, h, a, b, c, d);
+    mk_sha_round<5, true>(w, d, e, f, g, h, a, b, c);
+    mk_sha_round<6, true>(w, c, d, e, f, g, h, a, b);
+    mk_sha_round<7, true>(w, b, c, d, e, f, g, h, a);
+    mk_sha_8rounds<8, true>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<16, true>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<24, true>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<32, true>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<40, true>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<48, true>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<56, true>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<64, true>(w, a, b, c, d, e, f, g, h);
+    mk_sha_8rounds<72, true>(w, a, b, c, d, e, f, g, h);
+    mk_sha_finish(a, b, c, d, iv, s_out);
 }
 
 // Seed of index i for source 0 (ChaCha20) or 1 (counter). `upper` keeps the second

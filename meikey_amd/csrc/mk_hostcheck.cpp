@@ -29,6 +29,14 @@ void hc_seed(const u32 key[8], u32 source, u64 index, u32 seed[8]) {
 
 void hc_secret_scalar(const u32 seed[8], u32 s[8]) { mk_secret_scalar(seed, s); }
 
+// The counter-seed scalar through the per-search SHA-512 mid-state.
+void hc_secret_scalar_counter(const u32 key[8], u64 index, u32 s[8]) {
+    mk_sha_mid mid;
+    mk_sha_mid_init(key, mid);
+    const u64 t = ((u64)key[7] << 32 | key[6]) + index;
+    mk_secret_scalar_mid(mid, mk_be_word((u32)t, (u32)(t >> 32)), s);
+}
+
 void hc_fe_mul_limbs(const u32 f[10], const u32 g[10], u32 out[10]) {
     fe a, b;
     memcpy(a.v, f, 40);
@@ -114,10 +122,15 @@ int hc_lane(const u32* table, u32 w, const u32 key[8], u32 source, u64 first, u3
         fe xn, xd;
         mk_mul_base_frac(table, w, s, yn, yd, xn, xd);
     };
-    mk_lane_tile(K, key, source, first, sp, mul, [] {}, [&](u32 j, const u32 pk[8]) {
-        memcpy(ys + 8 * j, pk, 32);
-        cand[j] = mk_pk_match(pk, mt, true) ? 1 : 0;
-    });
+    mk_sha_mid mid;
+    mk_sha_mid_init(key, mid);
+    mk_lane_tile(
+        K, key, source, first, sp, mul, [] {},
+        [&](u32 j, const u32 pk[8]) {
+            memcpy(ys + 8 * j, pk, 32);
+            cand[j] = mk_pk_match(pk, mt, true) ? 1 : 0;
+        },
+        [&]() -> const mk_sha_mid& { return mid; });
     return 0;
 }
 

@@ -104,6 +104,7 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
                      const mk_serve_state* __restrict__ serve, u64* __restrict__ next_index) {
     __shared__ u64 lmask[64];
     __shared__ uint4 stage[2 * 8 * 64];
+    __shared__ mk_sha_mid smid;
     const u32 lane = threadIdx.x;
     const u64 lanes = (u64)gridDim.x * 64;
     const u64 gl = (u64)blockIdx.x * 64 + lane;
@@ -154,6 +155,14 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
         for (int i = 0; i < 8; i++) key[i] = dsc.key[i];
         const u64 first = tile_first + (u64)lane * K;
         const u32 stop_on_hit = dsc.stop_on_hit;
+        if (dsc.source == 1) {
+            // the counter seeds' SHA-512 mid-state, the same for the whole wave
+            mk_sha_mid m;
+            mk_sha_mid_init(key, m);
+            __syncthreads();
+            if (lane == 0) smid = m;
+            __syncthreads();
+        }
         mk_spill sp;
         sp.base = scratch + gl;
         sp.stride = lanes;
@@ -172,6 +181,17 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
                 __syncthreads();
                 mt.dfa = ldfa;
             }
+        };
+        // volatile: loaded where each key's hash needs it, not kept in registers
+        auto mid = [&]() {
+            const volatile mk_sha_mid* v = &smid;
+            mk_sha_mid m;
+            MK_UNROLL
+            for (int i = 0; i < 8; i++) m.v[i] = v->v[i];
+            m.w2 = v->w2;
+            m.w16 = v->w16;
+            m.w17 = v->w17;
+            return m;
         };
         mk_lane_tile(K, key, dsc.source, first, sp, mul, pre_sweep, [&](u32 jj, const u32 pk[8]) {
             const bool cand = mk_pk_match(pk, mt, true);
@@ -195,7 +215,7 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
                     atomicOr(&ctr->overflow, 1u);
                 }
             }
-        });
+        }, mid);
     }
 }
 

@@ -30,16 +30,26 @@ struct mk_spill {
 // Keys first .. first + K - 1: seed -> scalar -> (yn, yd) by mul(s, yn, yd); one
 // inversion of the product of the yd; then pre_sweep(); then, from the last key to the
 // first, y = yn / yd is compressed (x sign bit left 0) and on_key(j, pk_words) is called.
-template <class M, class P, class F>
+// Counter seeds (source 1) hash through the search's SHA-512 mid-state, which mid()
+// returns; the search kernel keeps it in LDS and reads it afresh for each key, so its
+// 22 registers are not held through the comb multiply.
+template <class M, class P, class F, class G>
 MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const mk_spill& sp, M mul, P pre_sweep,
-                         F on_key) {
+                         F on_key, G mid) {
     u32 upper[8];
     fe pre;
+    const u64 tail = (u64)key[7] << 32 | key[6];
     MK_NO_UNROLL
     for (u32 j = 0; j < K; j++) {
-        u32 seed[8], s[8];
-        mk_seed_at(key, source, first + j, upper, j == 0, seed);
-        mk_secret_scalar(seed, s);
+        u32 s[8];
+        if (source == 1) {
+            const u64 t = tail + first + j;  // bytes 24..31 of the counter seed (LE)
+            mk_secret_scalar_mid(mid(), mk_be_word((u32)t, (u32)(t >> 32)), s);
+        } else {
+            u32 seed[8];
+            mk_seed_at(key, source, first + j, upper, j == 0, seed);
+            mk_secret_scalar(seed, s);
+        }
         fe yn, yd;
         mul(s, yn, yd);
         pre = j == 0 ? yd : fe_mul(pre, yd);

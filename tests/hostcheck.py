@@ -22,6 +22,7 @@ def lib():
         P, U32, U64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
         L.hc_seed.argtypes = [P, U32, U64, P]
         L.hc_secret_scalar.argtypes = [P, P]
+        L.hc_secret_scalar_counter.argtypes = [P, U64, P]
         L.hc_fe_mul_limbs.argtypes = [P, P, P]
         L.hc_fe_sq_limbs.argtypes = [P, P]
         L.hc_fe_to_words.argtypes = [P, P]
@@ -57,6 +58,12 @@ def seed(key: bytes, source: int, index: int) -> bytes:
 def secret_scalar(seed_: bytes) -> bytes:
     out = (ctypes.c_uint32 * 8)()
     lib().hc_secret_scalar(words(seed_), out)
+    return wbytes(out)
+
+
+def secret_scalar_counter(key: bytes, index: int) -> bytes:
+    out = (ctypes.c_uint32 * 8)()
+    lib().hc_secret_scalar_counter(words(key), index, out)
     return wbytes(out)
 
 

@@ -111,7 +111,7 @@ def _spec(key, source, needle, cs, mode, first, count, **kw):
     (0, b"z", True, M.MODE_SUFFIX),
 ])
 def test_search_vs_oracle(gpu, oracle, source, needle, cs, mode):
-    key = random.Random(hash(needle) & 0xFFFF).randbytes(32)
+    key = random.Random(int.from_bytes(needle, "little")).randbytes(32)
     first = 1 << 20
     count = 1 << 20
     say(f"search {needle} source={source} mode={mode}")
@@ -123,6 +123,24 @@ def test_search_vs_oracle(gpu, oracle, source, needle, cs, mode):
     for i in want[:32]:
         assert pks[i] == oracle.derive(key, source, i, 1)
         assert [h.seed for h in hits if h.index == i][0] == oracle.seed(key, source, i)
+
+
+def test_low_security_counter_mode(gpu, oracle):
+    """Low-security mode (counter seeds through the per-search SHA-512 mid-state): exact
+    hit sets across the 2^32 index boundary and across a key tail that wraps past 2^64,
+    and every 2^24-th key of the 2^32 range re-derived against the oracle."""
+    key = random.Random(3).randbytes(32)
+    wrap = key[:24] + (2**64 - 3000).to_bytes(8, "little")
+    for k, first in [(key, (1 << 32) - (1 << 19)), (wrap, 0)]:
+        say(f"counter search first={first}")
+        hits, tries = gpu.batch_run([_spec(k, M.SEED_COUNTER, b"mk7", False, M.MODE_CONTAINS, first, 1 << 20)])
+        want = oracle.search(k, M.SEED_COUNTER, first, tries[0], b"mk7", False, M.MODE_CONTAINS)
+        assert [h.index for h in hits] == want and len(want) > 0
+        for h in hits[:16]:
+            assert h.seed == oracle.seed(k, M.SEED_COUNTER, h.index)
+            assert h.pk == oracle.derive(k, M.SEED_COUNTER, h.index, 1)
+    for i in range(0, 1 << 32, 1 << 24):
+        assert gpu.derive(key, M.SEED_COUNTER, i, 1) == oracle.derive(key, M.SEED_COUNTER, i, 1), i
 
 
 def test_sign_dependent_suffix(gpu, oracle):

@@ -22,6 +22,17 @@ def test_seed_and_scalar(oracle, hostcheck):
             assert hostcheck.secret_scalar(s) == oracle.secret_scalar(s)
 
 
+def test_counter_scalar_from_mid_state(oracle, hostcheck):
+    """The SHA-512 mid-state path (rounds 0-3 and W16/W17 shared by a search) equals the
+    full hash of each counter seed, including a key tail that wraps past 2^64."""
+    rng = random.Random(22)
+    keys = [rng.randbytes(32) for _ in range(3)] + [rng.randbytes(24) + b"\xff" * 8]
+    for key in keys:
+        for i in [0, 1, 2, 255, 2**32 - 1, 2**64 - 1] + [rng.getrandbits(64) for _ in range(20)]:
+            want = oracle.secret_scalar(oracle.seed(key, 1, i))
+            assert hostcheck.secret_scalar_counter(key, i) == want, (key.hex(), i)
+
+
 def _rand_limbs(rng, even_max, odd_max):
     return [rng.randrange(0, (even_max if i % 2 == 0 else odd_max) + 1) for i in range(10)]
 
