@@ -93,7 +93,7 @@ void mk_last_error(char* buf, size_t len);
 /* Number of HIP devices (device_count, gpu.rs:1127-1140). */
 mk_status mk_device_count(int* n);
 
-/* Default options (W = 24, 64 keys per thread, R25 limbs, 2 waves per SIMD): the fastest
+/* Default options (W = 24, 256 keys per thread, R25 limbs, 2 waves per SIMD): the fastest
  * setting measured on MI355X; the W = 24 table takes 11.8 GB of HBM. */
 void mk_default_options(mk_options* out);
 

@@ -379,7 +379,7 @@ class Launch:
 class Gpu:
     """One device with its comb table (the reference's Gpu + Kernel)."""
 
-    def __init__(self, ordinal: int = 0, width: int = 24, batch: int = 64, blocks: int = 2, limbs: int = LIMBS_R25):
+    def __init__(self, ordinal: int = 0, width: int = 24, batch: int = 256, blocks: int = 2, limbs: int = LIMBS_R25):
         self.options = Options(width, batch, limbs, blocks)
         h = ctypes.c_void_p()
         _check(lib().mk_open(ordinal, ctypes.byref(self.options), ctypes.byref(h)))

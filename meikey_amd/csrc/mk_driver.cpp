@@ -262,7 +262,7 @@ mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* resul
     };
     mk_options base;
     mk_default_options(&base);
-    // stage 1: comb widths at 64 keys per lane and 2 waves per SIMD
+    // stage 1: comb widths at the default keys per lane and waves per SIMD
     std::vector<mk_options> c1;
     for (uint32_t w : {16u, 20u, 22u, 24u}) {
         mk_options o = base;
@@ -273,7 +273,8 @@ mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* resul
     mk_status st = stage(c1, w1);
     // stage 2: keys per lane (inversion batch) at the best width
     std::vector<mk_options> c2;
-    for (uint32_t b : {32u, 128u, 256u}) {
+    for (uint32_t b : {32u, 64u, 128u, 256u}) {
+        if (b == w1.batch) continue;
         mk_options o = w1;
         o.batch = b;
         c2.push_back(o);

@@ -221,7 +221,7 @@ mk_status mk_device_count(int* n) {
 void mk_default_options(mk_options* out) {
     if (!out) return;
     out->width = 24;
-    out->batch = 64;
+    out->batch = 256;
     out->limbs = MK_LIMBS_R25;
     out->blocks = 2;
 }

@@ -53,9 +53,10 @@ struct mk_serve_state {
 
 // Launch-wide counters, cleared before each step.
 struct mk_counters {
-    u64 queue;      // next tile to hand out
+    u64 queue;      // next whole tile to hand out
+    u64 queue2;     // next tail unit to hand out (tiles cut into smaller units at the end)
     u32 n_cands;    // candidates appended (may exceed the buffer; the excess is dropped and flagged)
     u32 n_hits;     // exact hits appended
     u32 overflow;   // set when candidates or hits did not fit
-    u32 pad[3];
+    u32 pad;
 };

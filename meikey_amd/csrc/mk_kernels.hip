@@ -109,10 +109,27 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
     const u64 lanes = (u64)gridDim.x * 64;
     const u64 gl = (u64)blockIdx.x * 64 + lane;
     u32 cached = 0xffffffffu;
+    // Tail balancing (search launches): the last `gridDim.x` tiles are cut into U units of
+    // K / U keys per lane, claimed one by one from a second counter, so the waves finish
+    // within a fraction of a tile of each other. Units keep at least 64 keys per lane
+    // (the inversion is shared by a unit's keys).
+    const u32 U = (serve == nullptr && K >= 128 && K % 64 == 0) ? K / 64 : 1;
+    const u64 tail = U > 1 ? (total_tiles < gridDim.x ? total_tiles : gridDim.x) : 0;
+    const u64 big_tiles = total_tiles - tail;
     for (;;) {
-        u64 tile = 0;
-        if (lane == 0) tile = atomicAdd((unsigned long long*)&ctr->queue, 1ull);
-        tile = mk_uniform64(tile);
+        u64 tile = 0, sub = 0;
+        u32 Keff = K;
+        {
+            u64 claim = 0;
+            if (lane == 0) {
+                const u64 t = atomicAdd((unsigned long long*)&ctr->queue, 1ull);
+                claim = t < big_tiles ? t * U : big_tiles * U + atomicAdd((unsigned long long*)&ctr->queue2, 1ull);
+            }
+            claim = mk_uniform64(claim);
+            tile = claim / U;
+            sub = claim % U;
+            if (tile >= big_tiles) Keff = K / U;
+        }
         if (tile >= total_tiles) break;
         u32 search;
         u64 tile_first;
@@ -135,7 +152,7 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
             }
             const mk_seg sg = segs[lo];
             search = sg.search;
-            tile_first = sg.first + (tile - sg.tile_begin) * 64ull * K;
+            tile_first = sg.first + (tile - sg.tile_begin) * 64ull * K + sub * 64ull * Keff;
         }
         const mk_search_desc& dsc = descs[search];
         if (dsc.stop_on_hit && __atomic_load_n(&stop[search], __ATOMIC_RELAXED) != 0) continue;
@@ -153,7 +170,7 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
         u32 key[8];
         MK_UNROLL
         for (int i = 0; i < 8; i++) key[i] = dsc.key[i];
-        const u64 first = tile_first + (u64)lane * K;
+        const u64 first = tile_first + (u64)lane * Keff;
         const u32 stop_on_hit = dsc.stop_on_hit;
         if (dsc.source == 1) {
             // the counter seeds' SHA-512 mid-state, the same for the whole wave
@@ -193,7 +210,7 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
             m.w17 = v->w17;
             return m;
         };
-        mk_lane_tile(K, key, dsc.source, first, sp, mul, pre_sweep, [&](u32 jj, const u32 pk[8]) {
+        mk_lane_tile(Keff, key, dsc.source, first, sp, mul, pre_sweep, [&](u32 jj, const u32 pk[8]) {
             const bool cand = mk_pk_match(pk, mt, true);
             const u64 bal = __ballot(cand);
             if (bal == 0) return;
@@ -261,6 +278,7 @@ extern "C" __global__ void mk_reset_kernel(mk_counters* ctr, u32* stop, u32 nsto
     const u32 i = blockIdx.x * 64 + threadIdx.x;
     if (i == 0) {
         ctr->queue = 0;
+        ctr->queue2 = 0;
         ctr->n_cands = 0;
         ctr->n_hits = 0;
         ctr->overflow = 0;

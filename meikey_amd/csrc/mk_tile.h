@@ -59,14 +59,28 @@ MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const m
     }
     fe inv = fe_invert(pre);
     pre_sweep();
-    // 1/yd_j = inv * pre_(j-1) with inv = 1/pre_j; then inv <- inv * yd_j = 1/pre_(j-1)
+    // 1/yd_j = inv * pre_(j-1) with inv = 1/pre_j; then inv <- inv * yd_j = 1/pre_(j-1).
+    // Each key's three spilled values are loaded one key ahead, so their HBM latency
+    // overlaps the previous key's multiplications and matching.
+    fe n_yn = sp.load(3 * (K - 1)), n_yd = inv, n_pre = inv;
+    if (K > 1) {
+        n_yd = sp.load(3 * (K - 1) + 1);
+        n_pre = sp.load(3 * (K - 2) + 2);
+    }
     MK_NO_UNROLL
     for (u32 jj = K; jj-- > 0;) {
-        const fe yn = sp.load(3 * jj);
+        const fe yn = n_yn, yd = n_yd, pre_prev = n_pre;
+        if (jj > 0) {
+            n_yn = sp.load(3 * (jj - 1));
+            if (jj > 1) {
+                n_yd = sp.load(3 * (jj - 1) + 1);
+                n_pre = sp.load(3 * (jj - 2) + 2);
+            }
+        }
         fe zinv = inv;
         if (jj > 0) {
-            zinv = fe_mul(inv, sp.load(3 * (jj - 1) + 2));
-            inv = fe_mul(inv, sp.load(3 * jj + 1));
+            zinv = fe_mul(inv, pre_prev);
+            inv = fe_mul(inv, yd);
         }
         u32 pk[8];
         fe_to_words(fe_mul(yn, zinv), pk);

@@ -42,3 +42,38 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
         assert "missing" in str(e)
     else:
         raise AssertionError("expected MeikeyError")
+
+
+STRUCTS = {  # C struct -> ctypes mirror in meikey_amd
+    "mk_options": "Options", "mk_info": "Info", "mk_hit_rec": "HitRec", "mk_search_spec": "SearchSpec",
+    "mk_server_options": "ServerOptions", "mk_server_result": "ServerResult", "mk_stats": "Stats",
+    "mk_search_config": "SearchConfig", "mk_outcome": "Outcome", "mk_tune_result": "TuneResult",
+}
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """sizeof and every field offset of the C structs (gcc on include/meikey_hip.h) equal
+    the ctypes mirrors, so C, Python and a Rust #[repr(C)] binding agree on the bytes."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        import pytest
+        pytest.skip("gcc not installed")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "meikey_hip.h"', "int main(void) {"]
+    for c, py in STRUCTS.items():
+        cls = getattr(meikey_amd, py)
+        lines.append(f'printf("{c} size %zu\\n", sizeof({c}));')
+        for f in cls._fields_:
+            lines.append(f'printf("{c} {f[0]} %zu\\n", offsetof({c}, {f[0]}));')
+    lines += ["return 0; }"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for c, py in STRUCTS.items():
+        cls = getattr(meikey_amd, py)
+        assert int(got[f"{c} size"]) == ctypes.sizeof(cls), c
+        for f in cls._fields_:
+            assert int(got[f"{c} {f[0]}"]) == getattr(cls, f[0]).offset, (c, f[0])

@@ -125,6 +125,35 @@ def test_search_vs_oracle(gpu, oracle, source, needle, cs, mode):
         assert [h.seed for h in hits if h.index == i][0] == oracle.seed(key, source, i)
 
 
+def test_tail_units_match_whole_tiles(gpu, oracle):
+    """At 256 keys per lane the last tile per wave is cut into 64-key units claimed from a
+    second counter; hit sets over the same ranges equal those of whole 32-key tiles (and
+    the oracle on the smaller searches). The first search spans more tiles than the grid
+    has waves, so it runs through both the whole-tile and the unit claims."""
+    say("open gpu W=16 batch=256 blocks=1")
+    big = M.Gpu(0, width=16, batch=256, blocks=1)
+    try:
+        tile = 64 * 256
+        assert big.info().grid_waves * tile <= (1 << 24)
+        rng = random.Random(77)
+        specs = []
+        for count, needle, cs, mode in [((1 << 24) + 3 * tile, b"Kq7", True, M.MODE_CONTAINS),
+                                        (1 << 20, b"zz", False, M.MODE_SUFFIX),
+                                        (5 * tile, b"mk", False, M.MODE_CONTAINS)]:
+            specs.append(_spec(rng.randbytes(32), rng.choice([0, 1]), needle, cs, mode, rng.getrandbits(40), count))
+        hits, tries = big.batch_run(specs)
+        assert tries == [s["count"] for s in specs]
+        ref, _ = gpu.batch_run(specs)
+        assert [(h.search, h.index, h.pk) for h in hits] == [(h.search, h.index, h.pk) for h in ref]
+        for i in (1, 2):
+            s = specs[i]
+            want = oracle.search(s["key"], s["seed_source"], s["first"], s["count"], s["needle"], not s["fold"],
+                                 s["mode"])
+            assert [h.index for h in hits if h.search == i] == want and want
+    finally:
+        big.close()
+
+
 def test_low_security_counter_mode(gpu, oracle):
     """Low-security mode (counter seeds through the per-search SHA-512 mid-state): exact
     hit sets across the 2^32 index boundary and across a key tail that wraps past 2^64,
