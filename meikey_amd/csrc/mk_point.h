@@ -112,6 +112,13 @@ struct mk_entry {
     u32 w[MK_ENTRY_WORDS];
 };
 
+// Word of limb i of field f (0 = y+x, 1 = y-x, 2 = third field) in a table entry. Limbs
+// 0-7 of the three fields fill 16-byte chunks 0-1, 2-3 and 4-5; limbs 8-9 of y+x and y-x
+// share chunk 6 and those of the third field start chunk 7 (2 pad words). y+x and y-x
+// thus sit at symmetric chunk positions, and a negative digit (which swaps them) only
+// changes which chunks a lane reads.
+MK_DEV int mk_entry_word(int f, int i) { return i < 8 ? 8 * f + i : 24 + 2 * f + (i - 8); }
+
 #if defined(__HIPCC__)
 // One 128-byte entry: eight 16-byte vector loads of one cache line.
 MK_DEV void mk_load_entry(const u32* __restrict__ table, u64 row, i32 d, u32 cols, mk_entry& e) {
@@ -140,9 +147,9 @@ MK_DEV mk_niels mk_entry_niels(const mk_entry& e, i32 d) {
     mk_niels n;
     MK_UNROLL
     for (int i = 0; i < 10; i++) {
-        n.ypx.v[i] = e.w[i];
-        n.ymx.v[i] = e.w[10 + i];
-        n.xy2d.v[i] = e.w[20 + i];
+        n.ypx.v[i] = e.w[mk_entry_word(0, i)];
+        n.ymx.v[i] = e.w[mk_entry_word(1, i)];
+        n.xy2d.v[i] = e.w[mk_entry_word(2, i)];
     }
     if (d < 0) {
         const fe t = n.ypx;
@@ -206,16 +213,28 @@ MK_DEV void mk_stage_entry(const u32* __restrict__ table, u64 row, i32 d, u32 co
 }
 
 MK_DEV mk_niels mk_staged_niels(const uint4* stage, u32 b, u32 lane, i32 d) {
-    mk_entry e;
+    // chunk q of this lane: s[q * 64]. A negative digit reads y-x's chunks for y+x and
+    // the other way round, and the other half of chunk 6, so the swap costs no moves.
+    const bool neg = d < 0;
+    const uint4* s = stage + b * 8 * 64 + lane;
+    const uint4* pa = s + (neg ? 2 : 0) * 64;
+    const uint4* pb = s + (neg ? 0 : 2) * 64;
+    const uint4 a0 = pa[0], a1 = pa[64], b0 = pb[0], b1 = pb[64], c0 = s[4 * 64], c1 = s[5 * 64];
+    const u32* w6 = (const u32*)(s + 6 * 64);
+    const uint2 a2 = *(const uint2*)(w6 + (neg ? 2 : 0)), b2 = *(const uint2*)(w6 + (neg ? 0 : 2));
+    const uint2 c2 = *(const uint2*)(s + 7 * 64);
+    mk_niels n;
+    const u32 ya[10] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y};
+    const u32 yb[10] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y};
+    const u32 yc[10] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y};
     MK_UNROLL
-    for (int q = 0; q < MK_ENTRY_WORDS / 4; q++) {
-        const uint4 v = stage[(b * 8 + q) * 64 + lane];
-        e.w[4 * q] = v.x;
-        e.w[4 * q + 1] = v.y;
-        e.w[4 * q + 2] = v.z;
-        e.w[4 * q + 3] = v.w;
+    for (int i = 0; i < 10; i++) {
+        n.ypx.v[i] = ya[i];
+        n.ymx.v[i] = yb[i];
+        n.xy2d.v[i] = yc[i];
     }
-    return mk_entry_niels(e, d);
+    if (neg) n.xy2d = fe_neg(n.xy2d);
+    return n;
 }
 
 // mk_mul_base_frac with the gathers staged through LDS: the entry of row r + 1 is in
@@ -235,7 +254,6 @@ MK_DEV void mk_mul_base_frac_staged(const u32* __restrict__ table, u32 w, const 
         // the copy of this row's entry (issued one row ago) must have landed
         mk_wait_all();
         const mk_niels cur = mk_staged_niels(stage, row & 1, lane, dn);
-        d = dn;
         dn = mk_next_digit(g);
         // the other buffer was read into registers during the previous row
         mk_stage_entry(table, row + 1, dn, cols, stage, (row + 1) & 1);
@@ -325,7 +343,8 @@ MK_DEV void mk_row_bases(u32 w, mk_ext* out) {
 }
 
 // Entry (row, k) = k * base_row (+ 2^254 * B on the top row) as (y+x, y-x, 2dxy) in
-// canonical limbs, row 0 with 2xy as its third field; 32 words (2 pad words zero).
+// canonical limbs, row 0 with 2xy as its third field; 32 words laid out by
+// mk_entry_word (2 pad words zero).
 MK_DEV void mk_table_entry(u32 w, const mk_ext* bases, u32 row, u32 k, u32* out) {
     const u32 rows = mk_rows(w);
     const fe d2 = mk_const(MK_D2);
@@ -347,7 +366,7 @@ MK_DEV void mk_table_entry(u32 w, const mk_ext* bases, u32 row, u32 k, u32* out)
         u32 wds[8];
         fe_to_words(parts[f], wds);
         const fe c = fe_from_words(wds);  // canonical limbs
-        for (int i = 0; i < 10; i++) out[10 * f + i] = c.v[i];
+        for (int i = 0; i < 10; i++) out[mk_entry_word(f, i)] = c.v[i];
     }
     out[30] = 0;
     out[31] = 0;

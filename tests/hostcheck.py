@@ -73,6 +73,17 @@ def limbs_of(x: int):
     return [(x >> offs[i]) & ((1 << (offs[i + 1] - offs[i])) - 1) for i in range(10)]
 
 
+# Word index of limb i of field f (0 = y+x, 1 = y-x, 2 = third) in a 32-word table entry:
+# limbs 0-7 of the fields in words 0-7, 8-15, 16-23; limbs 8-9 in words 24-25, 26-27, 28-29;
+# words 30-31 are padding.
+def entry_word(f: int, i: int) -> int:
+    return 8 * f + i if i < 8 else 24 + 2 * f + (i - 8)
+
+
+def entry_limbs(words, f: int):
+    return [words[entry_word(f, i)] for i in range(10)]
+
+
 def value_of(limbs) -> int:
     offs = [0, 26, 51, 77, 102, 128, 153, 179, 204, 230]
     return sum(int(l) << o for l, o in zip(limbs, offs))

@@ -69,7 +69,7 @@ def test_derive_bulk_vs_oracle(gpu, oracle, source):
 
 
 def test_table_entries_vs_oracle(gpu, oracle):
-    from tests.hostcheck import value_of
+    from tests.hostcheck import entry_limbs, value_of
     w = 16
     rows, cols = 16, (1 << 15) + 1
     rng = random.Random(41)
@@ -78,9 +78,9 @@ def test_table_entries_vs_oracle(gpu, oracle):
     for row, k in picks:
         words = gpu.table_words(row, k)
         n = oracle.comb_entry_niels(w, row, k)
-        assert value_of(words[0:10]) == int.from_bytes(n[0:32], "little"), (row, k)
-        assert value_of(words[10:20]) == int.from_bytes(n[32:64], "little"), (row, k)
-        third = value_of(words[20:30])
+        assert value_of(entry_limbs(words, 0)) == int.from_bytes(n[0:32], "little"), (row, k)
+        assert value_of(entry_limbs(words, 1)) == int.from_bytes(n[32:64], "little"), (row, k)
+        third = value_of(entry_limbs(words, 2))
         xy2d = int.from_bytes(n[64:96], "little")
         assert third == (xy2d * T.inv(T.D) % T.P if row == 0 else xy2d), (row, k)
 

@@ -91,8 +91,19 @@ def test_digits(oracle, hostcheck):
 
 
 def _entry_ints(words):
-    from tests.hostcheck import value_of
-    return [value_of(words[10 * f:10 * f + 10]) for f in range(3)]
+    from tests.hostcheck import entry_limbs, value_of
+    return [value_of(entry_limbs(words, f)) for f in range(3)]
+
+
+def test_entry_layout_is_a_permutation():
+    """Every limb of the three fields has its own word, words 30-31 are free, and y+x /
+    y-x occupy mirrored 16-byte chunks (0-1 vs 2-3, halves of chunk 6)."""
+    from tests.hostcheck import entry_word
+    used = sorted(entry_word(f, i) for f in range(3) for i in range(10))
+    assert used == list(range(30))
+    for i in range(8):
+        assert entry_word(1, i) == entry_word(0, i) + 8
+    assert [entry_word(0, 8) // 4, entry_word(1, 8) // 4] == [6, 6]
 
 
 @pytest.mark.parametrize("w", [2, 3, 5, 8])
@@ -115,7 +126,9 @@ def test_table_matches_oracle(oracle, hostcheck, w):
             assert third == xy2d
         assert words[30] == 0 and words[31] == 0
         # canonical limbs
-        assert all(words[i] <= (MASK26 if (i % 10) % 2 == 0 else MASK25) for i in range(30))
+        from tests.hostcheck import entry_limbs
+        assert all(l <= (MASK26 if i % 2 == 0 else MASK25) for f in range(3)
+                   for i, l in enumerate(entry_limbs(words, f)))
 
 
 @pytest.mark.parametrize("w", [3, 7, 8])
