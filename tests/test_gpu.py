@@ -256,7 +256,8 @@ def test_regex_search_vs_oracle(gpu, oracle, text, cs):
     42) against the oracle's DFA walk, and every hit against Python re on the real key."""
     import re
     p = M.Pattern(text, cs, M.MODE_REGEX)
-    key = random.Random(hash((text, cs)) & 0xFFFF).randbytes(32)
+    import zlib
+    key = random.Random(zlib.crc32(text.encode()) * 2 + int(cs)).randbytes(32)
     first = 12345
     hits, tries = gpu.batch_run([p.spec(key, first, 1 << 19)])
     table = p.needle()[0]
@@ -265,6 +266,53 @@ def test_regex_search_vs_oracle(gpu, oracle, text, cs):
     rx = re.compile(text, re.ASCII | (0 if cs else re.IGNORECASE))
     for h in hits[:50]:
         assert rx.search(T.hay(h.pk)) is not None
+
+
+def test_large_dfa_from_global_memory(gpu, oracle):
+    """A DFA of more than 126 states does not fit the LDS staging buffers and is walked
+    from HBM; hit sets still equal the oracle's."""
+    p = M.Pattern("(ab|cd|ef|gh).{0,10}(12|34|56)", True, M.MODE_REGEX)
+    assert p.dfa_states > 126
+    key = random.Random(91).randbytes(32)
+    hits, tries = gpu.batch_run([p.spec(key, 777, 1 << 20)])
+    want = oracle.search(key, 0, 777, tries[0], p.needle()[0], True, M.MODE_REGEX)
+    assert [h.index for h in hits] == want and len(want) > 50
+
+
+def test_edge_inputs(gpu, oracle):
+    """Empty batch, zero-count and empty-needle searches next to a normal one, and a
+    43-character needle (the whole hay) that matches exactly one key."""
+    assert gpu.batch_run([]) == ([], [])
+    key = random.Random(92).randbytes(32)
+    specs = [_spec(key, 0, b"Ab", True, M.MODE_CONTAINS, 0, 0),
+             _spec(key, 0, b"", True, M.MODE_CONTAINS, 0, 1 << 16),
+             _spec(key, 0, b"Ab", True, M.MODE_CONTAINS, 5, 1 << 16)]
+    hits, tries = gpu.batch_run(specs)
+    assert tries[0] == 0 and tries[1] >= 1 << 16
+    assert {h.search for h in hits} <= {2}
+    assert [h.index for h in hits] == oracle.search(key, 0, 5, tries[2], b"Ab", True, M.MODE_CONTAINS)
+    target = 4242
+    hay = T.hay(oracle.derive(key, 0, target, 1)).encode()
+    assert len(hay) == 43
+    hits, _ = gpu.batch_run([_spec(key, 0, hay, True, M.MODE_CONTAINS, target - 1000, 4096)])
+    assert [h.index for h in hits] == [target]
+    hits, _ = gpu.batch_run([_spec(key, 0, hay.lower(), False, M.MODE_SUFFIX, target - 1000, 4096)])
+    assert [h.index for h in hits] == [target]
+
+
+def test_one_key_per_lane_and_eight_waves(oracle):
+    """The extremes of the launch shape: 1 key per lane (an inversion per key) and a grid
+    sized for 8 waves per SIMD (more waves than fit, queued by the hardware)."""
+    for batch, blocks in [(1, 1), (8, 8)]:
+        say(f"open gpu W=10 batch={batch} blocks={blocks}")
+        g = M.Gpu(0, width=10, batch=batch, blocks=blocks)
+        try:
+            key = random.Random(93 + batch).randbytes(32)
+            assert g.derive(key, 1, 99, 512) == oracle.derive(key, 1, 99, 512)
+            hits, tries = g.batch_run([_spec(key, 0, b"q", True, M.MODE_SUFFIX, 3, 1 << 17)])
+            assert [h.index for h in hits] == oracle.search(key, 0, 3, tries[0], b"q", True, M.MODE_SUFFIX)
+        finally:
+            g.close()
 
 
 def config4_specs(count):
