@@ -248,7 +248,9 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
     mk_gpu* g = new mk_gpu();
     g->device = ordinal;
     g->opt = opt;
+    void* bases = nullptr;  // row bases, needed only while the table is built
     auto bail = [&](mk_status s) {
+        if (bases) (void)hipFree(bases);
         mk_close(g);
         return s;
     };
@@ -274,7 +276,6 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
     // comb table, built on the device
     g->table_bytes = (u64)g->rows * g->cols * MK_ENTRY_WORDS * 4;
     MK_HIP_G(hipMalloc((void**)&g->table, g->table_bytes));
-    void* bases = nullptr;
     MK_HIP_G(hipMalloc(&bases, (size_t)(g->rows + 1) * MK_EXT_BYTES));
     MK_HIP_G(mk_k_table_bases(g->stream, opt.width, bases));
     const u64 entries = (u64)g->rows * g->cols;
@@ -283,6 +284,7 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
         MK_HIP_G(mk_k_table_fill(g->stream, opt.width, bases, e0, std::min(chunk, entries - e0), g->table));
     MK_HIP_G(hipStreamSynchronize(g->stream));
     MK_HIP_G(hipFree(bases));
+    bases = nullptr;
 
     g->scratch_bytes = g->lanes * opt.batch * 30 * 4;
     MK_HIP_G(hipMalloc((void**)&g->scratch, g->scratch_bytes));

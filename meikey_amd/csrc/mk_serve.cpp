@@ -290,6 +290,8 @@ double mk_server_now(const mk_server* s) { return s ? now_s(s) : 0.0; }
 
 mk_status mk_server_submit(mk_server* s, const mk_search_spec* spec, uint64_t* request) {
     if (!s || !spec) return mk_fail(MK_ERR_INVALID, "null pointer");
+    if (spec->seed_source != MK_SEED_CHACHA20 && spec->seed_source != MK_SEED_COUNTER)
+        return mk_fail(MK_ERR_INVALID, "unknown seed source %d", spec->seed_source);
     pending p;
     p.id = s->next_id++;
     p.spec = *spec;

@@ -432,6 +432,8 @@ def test_server_budget_and_cancel(gpu):
     assert big.dfa_states > 1
     table_bytes = len(big.needle()[0])
     with M.Server(gpu, max_slots=8, use_graph=True, dfa_budget=table_bytes + 16, step_keys=1 << 20) as srv:
+        with pytest.raises(M.MeikeyError):
+            srv.submit(big.spec(b"\x03" * 32, 0, 1 << 21, seed_source=7))
         a = srv.submit(big.spec(b"\x01" * 32, 0, 1 << 21))
         b = srv.submit(big.spec(b"\x02" * 32, 0, 1 << 21))
         active, queued = srv.step()
