@@ -14,6 +14,7 @@ the host cores over a bounded sample (which also checks the GPU's public keys of
 sample, and every key the server returned, bit for bit).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -24,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "secure-mode keys/s on 1 MI355X (single vs 64-search batched); p99 time-to-hit"
 STEP_KEYS = 1 << 28
+ctypes_u32, ctypes_u64, ctypes_buf = ctypes.c_uint32, ctypes.c_uint64, ctypes.create_string_buffer
 # MI355X integer VALU peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, one 32x32->64
 # multiply-add (v_mad_u64_u32) per lane per clock.
 PEAK_MUL = 256 * 4 * 16 * 2.4e9
@@ -287,11 +289,62 @@ def copy_calibration(nbytes: int) -> None:
     print(json.dumps({"copy_bytes": nbytes, "dispatches": 3}), flush=True)
 
 
+def reference_cpu_leg(key, needle, cores):
+    """The reference's own search kernel compiled as host C++ (oracle/_ref, from
+    kernel.cu), run on `cores` host threads over a bounded sample of the headline
+    workload, with the reference's default W = 16 comb table (built on the device and
+    repacked into the reference layout). Its public keys of sampled indices are compared
+    with the GPU's. None where the reference build is absent."""
+    import struct
+    import threading
+    import meikey_amd as M
+    from oracle import oracle as O
+    R = O.ref()
+    if R is None:
+        return None
+    g16 = M.Gpu(0, width=16, batch=8, blocks=1)
+    try:
+        info = g16.info()
+        table = O.ref_table_from_device(g16.table_words(0, 0, info.rows * info.cols, as_array=True))
+        kw = (ctypes_u32 * 8)(*struct.unpack("<8I", key))
+        batch = R.meikey_test_batch()
+        wrong = 0
+        for first in (0, 12345, (1 << 31) + 7, (1 << 32) - batch):
+            out = ctypes_buf(32 * batch)
+            R.meikey_test_derive_batch(table, 16, kw, 0, first, out)
+            if out.raw != g16.derive(key, 0, first, batch):
+                wrong += 1
+        nthreads, iters = 128, 16  # one call covers 128 x 16 x 32 = 65536 candidates
+        per_call = nthreads * iters * batch
+        calls_per_core = 4
+        def work(c):
+            for j in range(calls_per_core):
+                res = (ctypes_u64 * 2)()
+                base = (c * calls_per_core + j) * (1 << 26)
+                R.meikey_test_search(nthreads, table, 16, kw, 0, needle, len(needle), 0, 1, base, iters, 0, res)
+        ts = [threading.Thread(target=work, args=(c,)) for c in range(cores)]
+        t0 = time.perf_counter()
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        dt = time.perf_counter() - t0
+        keys = cores * calls_per_core * per_call
+        return {"value": keys / dt, "unit": "keys/s", "cores": cores, "kind": "reference",
+                "sample": f"{cores} threads x {calls_per_core} runs of the reference kernel's host build "
+                          f"(meikey_test_search, 128 threads x 16 iterations x 32 keys = {per_call} candidates "
+                          f"each, W = 16 table), the headline key and suffix",
+                "checked_keys": 4 * batch, "keys_differing_from_gpu": wrong}
+    finally:
+        g16.close()
+
+
 def cpu_baseline(gpu, key, needle, served=(), low=None):
-    """The CPU oracle (a scalar C port of the reference's per-key algorithm) on this
-    host's cores over a bounded sample of the same workload; the GPU's public keys of
-    the same sample, every key the server returned, and (low-security leg) the keys of
-    every 2^20-th counter-seed index are checked against it."""
+    """CPU baselines on this host's cores over bounded samples of the same workload: the
+    reference's kernel compiled for the host (kind "reference", where oracle/_ref was
+    built) and the CPU oracle (a scalar C port, kind "port"). The GPU's public keys of
+    the oracle's sample, every key the server returned and (low-security leg) the keys of
+    every 2^20-th counter-seed index are checked against the oracle."""
     from oracle import oracle as O
 
     cores = min(os.cpu_count() or 1, 16)
@@ -317,13 +370,21 @@ def cpu_baseline(gpu, key, needle, served=(), low=None):
             low_checked += 1
             if gpu.derive(lkey, 1, i, 1) != O.derive(lkey, 1, i, 1):
                 low_wrong += 1
-    return {"value": chunks * per / dt, "unit": "keys/s", "cores": cores, "kind": "port",
-            "served_keys_checked": len(served), "served_keys_wrong": bad_served,
-            "low_security_keys_checked": low_checked, "low_security_keys_wrong": low_wrong,
+    port = {"value": chunks * per / dt, "unit": "keys/s", "cores": cores, "kind": "port",
             "sample": f"{chunks} x {per} consecutive candidates spread over the 2^32 range, derive + suffix match "
                       f"with the C oracle; GPU public keys of the same 2^20 candidates compared: "
-                      f"{'all equal' if mismatches == 0 else f'{mismatches} chunks differ'}",
-            "checked_keys": chunks * per, "mismatched_chunks": mismatches}
+                      f"{'all equal' if mismatches == 0 else f'{mismatches} chunks differ'}"}
+    checks = {"served_keys_checked": len(served), "served_keys_wrong": bad_served,
+              "low_security_keys_checked": low_checked, "low_security_keys_wrong": low_wrong,
+              "checked_keys": chunks * per, "mismatched_chunks": mismatches}
+    # the reference's own kernel (host build) is the baseline where it was built; the
+    # port's rate is reported beside it
+    ref = reference_cpu_leg(key, needle, cores)
+    out = dict(ref) if ref else dict(port)
+    if ref:
+        out["port"] = port
+    out.update(checks)
+    return out
 
 
 if __name__ == "__main__":

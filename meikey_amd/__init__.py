@@ -452,9 +452,14 @@ class Gpu:
         _check(lib().mk_last_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
-    def table_words(self, row: int, k: int, count: int = 1) -> List[int]:
+    def table_words(self, row: int, k: int, count: int = 1, as_array: bool = False):
+        """Raw table entries from (row, k) on, 32 words each (a list, or with as_array a
+        numpy array of shape (count, 32))."""
         buf = (ctypes.c_uint32 * (32 * count))()
         _check(lib().mk_table_read(self._h, row, k, count, buf))
+        if as_array:
+            import numpy as np
+            return np.frombuffer(buf, dtype=np.uint32).reshape(count, 32).copy()
         return list(buf)
 
     def batch_run(self, specs: Sequence[dict], cap: int = 1 << 20, as_array: bool = False):

@@ -56,6 +56,64 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
+_ref = None
+
+
+def ref():
+    """The reference's own kernel compiled as host C++ (oracle/_ref, built by
+    `make ref` from the meikey sources), or None where it was not built. Its functions
+    (kernel.cu:2083-2208 test ABI, prefix meikey_test_: 10 limbs, 32 keys per thread,
+    packed 24-word table entries) pin the oracle against the reference itself."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF):
+            return None
+        L = ctypes.CDLL(REF)
+        vp, u32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
+        L.meikey_test_seed.argtypes = [vp, u64, vp]
+        L.meikey_test_counter_seed.argtypes = [vp, u64, vp]
+        L.meikey_test_secret.argtypes = [vp, vp]
+        L.meikey_test_secret_mid.argtypes = [vp, u64, vp]
+        L.meikey_test_digits.argtypes = [vp, u32, vp]
+        L.meikey_test_derive_batch.argtypes = [vp, u32, vp, u32, u64, vp]
+        L.meikey_test_contains.argtypes = [vp, vp, u32, u32, u32]
+        L.meikey_test_contains_unsigned.argtypes = [vp, vp, u32, u32, u32]
+        L.meikey_test_tail_b64.argtypes = [vp, vp]
+        L.meikey_test_search.argtypes = [u64, vp, u32, vp, u32, vp, u32, u32, u32, u64, u32, u32, vp]
+        L.meikey_test_batch.restype = u32
+        L.meikey_test_entry_words.restype = u32
+        _ref = L
+    return _ref
+
+
+# Limb offsets of the ten-limb radix-2^25.5 form and the word of limb i of field f in a
+# 32-word entry of this repository's device table (limbs 0-7 of the fields in words
+# 0-7 / 8-15 / 16-23, limbs 8-9 in words 24-25 / 26-27 / 28-29).
+_LIMB_OFFSETS = [0, 26, 51, 77, 102, 128, 153, 179, 204, 230]
+
+
+def _entry_word(f: int, i: int) -> int:
+    return 8 * f + i if i < 8 else 24 + 2 * f + (i - 8)
+
+
+def ref_table_from_device(words):
+    """The reference's packed table (24 words per entry: three canonical 8-word LE
+    values, row 0's third field 2xy, gpu.rs:1455-1480) from the device table's entries
+    (numpy uint32, shape (entries, 32), canonical limbs)."""
+    import numpy as np
+    n = words.shape[0]
+    out = np.zeros((n, 24), dtype=np.uint64)
+    for f in range(3):
+        for i, off in enumerate(_LIMB_OFFSETS):
+            limb = words[:, _entry_word(f, i)].astype(np.uint64) << np.uint64(off % 32)
+            w = off // 32
+            out[:, 8 * f + w] |= limb & np.uint64(0xFFFFFFFF)
+            if w + 1 < 8:
+                out[:, 8 * f + w + 1] |= limb >> np.uint64(32)
+    flat = out.astype(np.uint32).reshape(-1)
+    return (ctypes.c_uint32 * flat.size).from_buffer_copy(flat.tobytes())
+
+
 def _buf(n: int):
     return ctypes.create_string_buffer(n)
 
