@@ -305,7 +305,9 @@ def reference_cpu_leg(key, needle, cores):
     g16 = M.Gpu(0, width=16, batch=8, blocks=1)
     try:
         info = g16.info()
-        table = O.ref_table_from_device(g16.table_words(0, 0, info.rows * info.cols, as_array=True))
+        import numpy as np
+        rows = [g16.table_words(r, 0, info.cols, as_array=True) for r in range(info.rows)]
+        table = O.ref_table_from_device(np.concatenate(rows))
         kw = (ctypes_u32 * 8)(*struct.unpack("<8I", key))
         batch = R.meikey_test_batch()
         wrong = 0
@@ -334,7 +336,7 @@ def reference_cpu_leg(key, needle, cores):
                 "sample": f"{cores} threads x {calls_per_core} runs of the reference kernel's host build "
                           f"(meikey_test_search, 128 threads x 16 iterations x 32 keys = {per_call} candidates "
                           f"each, W = 16 table), the headline key and suffix",
-                "checked_keys": 4 * batch, "keys_differing_from_gpu": wrong}
+                "reference_keys_checked": 4 * batch, "reference_keys_differing_from_gpu": wrong}
     finally:
         g16.close()
 
