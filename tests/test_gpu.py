@@ -268,6 +268,39 @@ def test_regex_search_vs_oracle(gpu, oracle, text, cs):
         assert rx.search(T.hay(h.pk)) is not None
 
 
+def test_full_scale_hits_are_exact(gpu, oracle):
+    """2^30 candidates in one launch at the default 256 keys per lane: every reported hit is
+    re-derived and re-matched on the CPU, and four random 2^16-key windows of the range
+    have exactly the oracle's hit sets (so nothing is missed there either)."""
+    say("open gpu defaults for 2^30 candidates")
+    big = M.Gpu(0)
+    try:
+        key = random.Random(95).randbytes(32)
+        needle = b"mk7q"
+        hits, tries = big.batch_run([_spec(key, 0, needle, False, M.MODE_CONTAINS, 1 << 33, 1 << 30)],
+                                    as_array=True)
+        assert tries[0] == 1 << 30
+        n = len(hits)
+        per_position = 1.0
+        for c in needle.decode():
+            per_position *= 2 / 64 if c.isalpha() else 1 / 64  # case-insensitive letters: 2 of 64
+        expect = (1 << 30) * (43 - len(needle) + 1) * per_position
+        assert 0.7 * expect < n < 1.3 * expect, (n, expect)
+        idx = hits["index"].tolist()
+        assert idx == sorted(idx) and len(set(idx)) == n
+        for i in range(0, n, max(1, n // 2000)):
+            pk = oracle.derive(key, 0, idx[i], 1)
+            assert pk == hits["pk"][i].tobytes()
+            assert oracle.match(oracle.public_b64(pk), needle, False, M.MODE_CONTAINS)
+        rng = random.Random(96)
+        for _ in range(4):
+            lo = (1 << 33) + rng.randrange((1 << 30) - (1 << 16))
+            want = oracle.search(key, 0, lo, 1 << 16, needle, False, M.MODE_CONTAINS)
+            assert [i for i in idx if lo <= i < lo + (1 << 16)] == want
+    finally:
+        big.close()
+
+
 def test_large_dfa_from_global_memory(gpu, oracle):
     """A DFA of more than 126 states does not fit the LDS staging buffers and is walked
     from HBM; hit sets still equal the oracle's."""
