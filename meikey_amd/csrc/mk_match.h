@@ -37,20 +37,38 @@ MK_DEV u32 mk_tail_value(const u32 pk[8], int pos) {
     return (n >> (18 - 6 * k)) & 63u;
 }
 
-// Substring (suffix = false) or suffix match by shift-and over tail positions 1..43.
-// mask: 64 entries; len in 1..42.
-MK_DEV bool mk_bitap_match(const u32 pk[8], const u64* mask, u32 len, bool suffix, bool sign_unknown) {
-    const u64 last = 1ull << (len - 1);
-    u64 D = 0, seen = 0;
+// Shift-and over tail positions From..43; returns the final state D and, in seen, the OR
+// of all states. Bit k of D at position p is set when needle[0..k] ends at p.
+template <int From>
+MK_DEV u64 mk_bitap_run(const u32 pk[8], const u64* mask, bool sign_unknown, u64& seen) {
+    u64 D = 0;
+    seen = 0;
     MK_UNROLL
-    for (int pos = MK_SEARCH_FROM; pos < MK_TAIL_B64; pos++) {
+    for (int pos = From; pos < MK_TAIL_B64; pos++) {
         const u32 v = mk_tail_value(pk, pos);
         u64 m = mask[v];
         if (pos == MK_SIGN_CHAR && sign_unknown) m |= mask[v | 2u];
         D = ((D << 1) | 1ull) & m;
         seen |= D;
     }
-    return ((suffix ? D : seen) & last) != 0;
+    return D;
+}
+
+// Substring (suffix = false) or suffix match by shift-and over tail positions 1..43.
+// mask: 64 entries; len in 1..43. A suffix of len characters can only start at
+// position 44 - len, so the automaton starts at 36 for len <= 8 and at 28 for len <= 16
+// (the final state is the same as from position 1).
+MK_DEV bool mk_bitap_match(const u32 pk[8], const u64* mask, u32 len, bool suffix, bool sign_unknown) {
+    const u64 last = 1ull << (len - 1);
+    u64 seen = 0;
+    if (suffix) {
+        const u64 D = len <= 8    ? mk_bitap_run<MK_TAIL_B64 - 8>(pk, mask, sign_unknown, seen)
+                      : len <= 16 ? mk_bitap_run<MK_TAIL_B64 - 16>(pk, mask, sign_unknown, seen)
+                                  : mk_bitap_run<MK_SEARCH_FROM>(pk, mask, sign_unknown, seen);
+        return (D & last) != 0;
+    }
+    mk_bitap_run<MK_SEARCH_FROM>(pk, mask, sign_unknown, seen);
+    return (seen & last) != 0;
 }
 
 MK_DEV u32 mk_dfa_next(const u16* dfa, u32 s, u32 col) { return dfa[s * 65u + col]; }

@@ -155,11 +155,15 @@ def test_full_pk_and_lane(oracle, hostcheck, w):
 def test_bitap_matcher_vs_oracle(oracle, hostcheck):
     rng = random.Random(24)
     alpha = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
-    for _ in range(4000):
+    for _ in range(6000):
         pk = rng.randbytes(32)
         b64 = T.public_b64(pk)
-        L = rng.choice([1, 2, 3, 5, 8, 42, 43])
-        if rng.random() < 0.6:
+        # lengths on both sides of the suffix automaton's start points (8 / 16)
+        L = rng.choice([1, 2, 3, 5, 7, 8, 9, 12, 16, 17, 30, 42, 43])
+        r = rng.random()
+        if r < 0.4:
+            needle = b64[68 - L:]  # a true suffix of the key
+        elif r < 0.7:
             st = rng.randrange(25, 69 - L)
             needle = b64[st:st + L]
         else:
