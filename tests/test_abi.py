@@ -77,3 +77,16 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert int(got[f"{c} size"]) == ctypes.sizeof(cls), c
         for f in cls._fields_:
             assert int(got[f"{c} {f[0]}"]) == getattr(cls, f[0]).offset, (c, f[0])
+
+
+def test_no_device_fails_loudly():
+    """Without a GPU every search entry point fails with a status and a message; there is
+    no CPU fallback. (Skipped where a device is visible.)"""
+    import pytest
+    if meikey_amd.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    with pytest.raises(meikey_amd.MeikeyError) as e:
+        meikey_amd.Gpu(0)
+    assert e.value.status == 2 and "no HIP device" in str(e.value)
+    with pytest.raises(meikey_amd.MeikeyError):
+        meikey_amd.fastest_staged(0)
