@@ -301,6 +301,41 @@ def test_full_scale_hits_are_exact(gpu, oracle):
         big.close()
 
 
+def test_overflow_is_an_error_not_a_short_list(gpu):
+    """More candidates than the device lists hold (2^20) in a search that does not stop
+    on its first hit is MK_ERR_OVERFLOW, never a silently truncated hit set; the same
+    search with stop_on_hit succeeds."""
+    key = random.Random(97).randbytes(32)
+    spec = _spec(key, 0, b"A", False, M.MODE_CONTAINS, 0, 1 << 23)  # ~2^23 * 43/32 candidates
+    with pytest.raises(M.MeikeyError) as e:
+        gpu.batch_run([spec])
+    assert e.value.status == 7
+    hits, _ = gpu.batch_run([dict(spec, stop_on_hit=True)])
+    assert len(hits) >= 1
+
+
+def test_searcher_counter_mode_and_served_large_dfa(gpu, oracle):
+    """The Searcher in low-security mode returns a true hit; a served request whose DFA
+    (> 126 states) is walked from HBM returns the same key alone as the oracle finds."""
+    key = random.Random(98).randbytes(32)
+    out = gpu.search(M.Pattern("Qz", True, M.MODE_SUFFIX), M.SEED_COUNTER, key=key)
+    assert out.found == 1 and bytes(out.pk) == oracle.derive(key, 1, out.index, 1)
+    assert bytes(out.seed) == oracle.seed(key, 1, out.index)
+    p = M.Pattern("(ab|cd|ef|gh).{0,10}(12|34|56)", True, M.MODE_REGEX)
+    with M.Server(gpu, max_slots=4, use_graph=True, step_keys=1 << 22) as srv:
+        rid = srv.submit(p.spec(key, 0, 0, M.SEED_CHACHA20, stop_on_hit=True))
+        done = []
+        for _ in range(400):
+            srv.step()
+            done += srv.poll()
+            if done:
+                break
+        assert done and done[0].request == rid and done[0].status == 1
+        r = done[0]
+        first = oracle.search(key, 0, 0, r.index + 1, p.needle()[0], True, M.MODE_REGEX)
+        assert first and first[0] == r.index and r.pk == oracle.derive(key, 0, r.index, 1)
+
+
 def test_large_dfa_from_global_memory(gpu, oracle):
     """A DFA of more than 126 states does not fit the LDS staging buffers and is walked
     from HBM; hit sets still equal the oracle's."""
