@@ -109,11 +109,12 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
     const u64 lanes = (u64)gridDim.x * 64;
     const u64 gl = (u64)blockIdx.x * 64 + lane;
     u32 cached = 0xffffffffu;
-    // Tail balancing (search launches): the last `gridDim.x` tiles are cut into U units of
-    // K / U keys per lane, claimed one by one from a second counter, so the waves finish
-    // within a fraction of a tile of each other. Units keep at least 64 keys per lane
-    // (the inversion is shared by a unit's keys).
-    const u32 U = (serve == nullptr && K >= 128 && K % 64 == 0) ? K / 64 : 1;
+    // Tail balancing: the last `gridDim.x` tiles are cut into U units of K / U keys per
+    // lane, claimed one by one from a second counter, so the waves finish within a
+    // fraction of a tile of each other. Units keep at least 64 keys per lane (the
+    // inversion is shared by a unit's keys). A serving step of one tile per wave is
+    // therefore claimed entirely in units.
+    const u32 U = (K >= 128 && K % 64 == 0) ? K / 64 : 1;
     const u64 tail = U > 1 ? (total_tiles < gridDim.x ? total_tiles : gridDim.x) : 0;
     const u64 big_tiles = total_tiles - tail;
     for (;;) {
@@ -137,10 +138,12 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
             // serving: round-robin over the active slots, indices from the slot's counter
             const u32 n_active = serve->n_active;
             if (n_active == 0) break;
-            search = serve->slots[tile % n_active];
+            // claim number: whole tiles first, then the units of the tail
+            const u64 nth = tile < big_tiles ? tile : big_tiles + (tile - big_tiles) * U + sub;
+            search = serve->slots[nth % n_active];
             if (__atomic_load_n(&stop[search], __ATOMIC_RELAXED) != 0) continue;
             u64 f = 0;
-            if (lane == 0) f = atomicAdd((unsigned long long*)&next_index[search], 64ull * K);
+            if (lane == 0) f = atomicAdd((unsigned long long*)&next_index[search], 64ull * Keff);
             tile_first = mk_uniform64(f);
         } else {
             // segment holding this tile (binary search; segments are few and in order)
