@@ -73,9 +73,11 @@ struct mk_launch {
     u32* stop = nullptr;
 };
 
+// One launch: reset, search, confirm. On return `hits` points at the min(n_hits, kHitCap)
+// exact hits in the device's pinned host buffer (valid until the next launch on g).
 mk_status mk_run_step(mk_gpu* g, const mk_search_desc* descs, const u64* masks, const u16* dfas, const mk_seg* segs,
-                   u32 nsegs, u64 total_tiles, u32* stop, u32 nstop, std::vector<mk_hit>& out, u32& n_cands,
-                   u32& n_hits, u32& overflow) {
+                      u32 nsegs, u64 total_tiles, u32* stop, u32 nstop, const mk_hit*& hits, u32& n_cands,
+                      u32& n_hits, u32& overflow) {
     MK_HIP(mk_k_reset(g->stream, g->ctr, stop, nstop, nullptr));
     MK_HIP(hipEventRecord(g->ev[0], g->stream));
     if (total_tiles > 0)
@@ -94,28 +96,36 @@ mk_status mk_run_step(mk_gpu* g, const mk_search_desc* descs, const u64* masks, 
     n_hits = g->h_ctr->n_hits;
     overflow = g->h_ctr->overflow;
     const u32 n = std::min(n_hits, kHitCap);
-    out.resize(n);
     if (n) {
         MK_HIP(hipMemcpyAsync(g->h_hits, g->hits, sizeof(mk_hit) * n, hipMemcpyDeviceToHost, g->stream));
         MK_HIP(hipStreamSynchronize(g->stream));
-        memcpy(out.data(), g->h_hits, sizeof(mk_hit) * n);
     }
+    hits = g->h_hits;
     return MK_OK;
 }
 
 namespace {
 
-void to_records(std::vector<mk_hit>& hs, mk_hit_rec* recs, u32 cap) {
-    std::sort(hs.begin(), hs.end(), [](const mk_hit& a, const mk_hit& b) {
+// The n exact hits as caller records, ordered by (search, index). The order is
+// sorted on 16-byte keys and the 80-byte records are gathered once.
+void to_records(const mk_hit* hs, u32 n, mk_hit_rec* recs, u32 cap) {
+    struct key {
+        u64 index;
+        u32 search, pos;
+    };
+    std::vector<key> order(n);
+    for (u32 i = 0; i < n; i++) order[i] = {hs[i].index, hs[i].search, i};
+    std::sort(order.begin(), order.end(), [](const key& a, const key& b) {
         return a.search != b.search ? a.search < b.search : a.index < b.index;
     });
-    const size_t n = std::min<size_t>(hs.size(), cap);
-    for (size_t i = 0; i < n; i++) {
-        recs[i].index = hs[i].index;
-        recs[i].search = hs[i].search;
+    const size_t m = std::min<size_t>(n, cap);
+    for (size_t i = 0; i < m; i++) {
+        const mk_hit& h = hs[order[i].pos];
+        recs[i].index = h.index;
+        recs[i].search = h.search;
         recs[i].pad = 0;
-        mk_words_to_bytes(hs[i].seed, recs[i].seed);
-        mk_words_to_bytes(hs[i].pk, recs[i].pk);
+        mk_words_to_bytes(h.seed, recs[i].seed);
+        mk_words_to_bytes(h.pk, recs[i].pk);
     }
 }
 
@@ -399,13 +409,13 @@ mk_status mk_launch_run(mk_launch* l, uint64_t base, uint32_t iters, mk_hit_rec*
     seg.tile_end = (u64)g->grid * iters;
     seg.search = 0;
     MK_HIP(hipMemcpyAsync(l->segs, &seg, sizeof seg, hipMemcpyHostToDevice, g->stream));
-    std::vector<mk_hit> hs;
+    const mk_hit* hs = nullptr;
     u32 n_cands = 0, nh = 0, overflow = 0;
     mk_status st = mk_run_step(g, l->descs, l->masks, l->dfas, l->segs, 1, seg.tile_end, l->stop, 1, hs, n_cands, nh,
-                            overflow);
+                               overflow);
     if (st != MK_OK) return st;
     *n_hits = nh;
-    to_records(hs, hits, cap);
+    to_records(hs, std::min(nh, kHitCap), hits, cap);
     // a stop-on-hit search needs one hit, not the full set: lists that overflowed while
     // the tiles already in flight finished are not an error
     if ((overflow || nh > cap) && !l->stop_on_hit)
@@ -537,14 +547,14 @@ mk_status mk_batch_run(mk_gpu* g, const mk_search_spec* specs, uint32_t n, mk_hi
     MK_HIP(hipMemcpyAsync(g->dfas.p, dfas.data(), sizeof(u16) * dfas.size(), hipMemcpyHostToDevice, g->stream));
     if (!live.empty())
         MK_HIP(hipMemcpyAsync(g->segs.p, live.data(), sizeof(mk_seg) * live.size(), hipMemcpyHostToDevice, g->stream));
-    std::vector<mk_hit> hs;
+    const mk_hit* hs = nullptr;
     u32 n_cands = 0, nh = 0, overflow = 0;
     st = mk_run_step(g, (const mk_search_desc*)g->descs.p, (const u64*)g->masks.p, (const u16*)g->dfas.p,
-                  (const mk_seg*)g->segs.p, (u32)live.size(), live.empty() ? 0 : tiles, (u32*)g->stop.p, n, hs,
-                  n_cands, nh, overflow);
+                     (const mk_seg*)g->segs.p, (u32)live.size(), live.empty() ? 0 : tiles, (u32*)g->stop.p, n, hs,
+                     n_cands, nh, overflow);
     if (st != MK_OK) return st;
     *n_hits = nh;
-    to_records(hs, hits, cap);
+    to_records(hs, std::min(nh, kHitCap), hits, cap);
     bool all_stop = true;
     for (u32 i = 0; i < n; i++) all_stop = all_stop && specs[i].stop_on_hit;
     if ((overflow || nh > cap) && !all_stop)

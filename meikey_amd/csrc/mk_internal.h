@@ -70,7 +70,7 @@ struct mk_gpu {
 // One step: clear the counters, run the search kernel over the segments (or the serving
 // slots), confirm the candidates, and copy counters and exact hits back (unsorted).
 mk_status mk_run_step(mk_gpu* g, const mk_search_desc* descs, const u64* masks, const u16* dfas, const mk_seg* segs,
-                      u32 nsegs, u64 total_tiles, u32* stop, u32 nstop, std::vector<mk_hit>& out, u32& n_cands,
+                      u32 nsegs, u64 total_tiles, u32* stop, u32 nstop, const mk_hit*& hits, u32& n_cands,
                       u32& n_hits, u32& overflow);
 
 void mk_words_to_bytes(const u32 w[8], uint8_t out[32]);
