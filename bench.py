@@ -346,7 +346,7 @@ def cpu_baseline(gpu, key, needle, served=(), low=None):
     reference's kernel compiled for the host (kind "reference", where oracle/_ref was
     built) and the CPU oracle (a scalar C port, kind "port"). The GPU's public keys of
     the oracle's sample, every key the server returned and (low-security leg) the keys of
-    every 2^20-th counter-seed index are checked against the oracle."""
+    every 2^16-th counter-seed index are checked against the oracle."""
     from oracle import oracle as O
 
     cores = min(os.cpu_count() or 1, 16)
@@ -367,11 +367,16 @@ def cpu_baseline(gpu, key, needle, served=(), low=None):
             bad_served += 1
     low_checked = low_wrong = 0
     if low is not None:
+        # every 2^16-th counter-seed index of the 2^32 range: 65,536 keys, the GPU's one by
+        # one against the oracle's (computed on the host threads)
+        from concurrent.futures import ThreadPoolExecutor
         lkey, _ = low
-        for i in range(0, 1 << 32, 1 << 20):
-            low_checked += 1
-            if gpu.derive(lkey, 1, i, 1) != O.derive(lkey, 1, i, 1):
-                low_wrong += 1
+        idx = list(range(0, 1 << 32, 1 << 16))
+        gpu_pks = [gpu.derive(lkey, 1, i, 1) for i in idx]
+        with ThreadPoolExecutor(cores) as ex:
+            cpu_pks = list(ex.map(lambda i: O.derive(lkey, 1, i, 1), idx, chunksize=256))
+        low_checked = len(idx)
+        low_wrong = sum(1 for a, b in zip(gpu_pks, cpu_pks) if a != b)
     port = {"value": chunks * per / dt, "unit": "keys/s", "cores": cores, "kind": "port",
             "sample": f"{chunks} x {per} consecutive candidates spread over the 2^32 range, derive + suffix match "
                       f"with the C oracle; GPU public keys of the same 2^20 candidates compared: "
