@@ -250,6 +250,7 @@ def test_config1_full_hit_set(gpu, oracle):
 @pytest.mark.parametrize("text,cs", [
     ("ab", True), ("^[A-P]a|z$", False), ("(ab){2,}", True), ("\\b9", True), ("[0-9][A-F]x", False),
     ("Q.$", True), ("[a-z]{2}[0-9]$", True), ("^[A-D][0-9]", True), ("a.b", False),
+    ("(?i)abc", True), ("\\d{3}$", True), ("(ab|c(d|e))+f", True), ("[^A-Za-z]{4}", True), ("x{1,3}y$", False),
 ])
 def test_regex_search_vs_oracle(gpu, oracle, text, cs):
     """DFA matching in the search kernel (LDS-resident table, sign fork at tail position
