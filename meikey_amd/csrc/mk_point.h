@@ -32,10 +32,10 @@ MK_DEV mk_ext mk_add_niels(const mk_ext& p, const mk_niels& q) {
     mk_ext r;
     // t (limbs < 2^28) only ever on the first operand; x and z are each the second
     // operand of two products, so their 19x multiples are computed once
+    r.t = fe_mul(y, x);
     r.x = fe_mul(t, x);
     r.y = fe_mul(y, z);
     r.z = fe_mul(t, z);
-    r.t = fe_mul(y, x);
     return r;
 }
 
