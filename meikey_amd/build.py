@@ -1,6 +1,7 @@
 """Build the gfx950 search library and its CPU test build, in-tree.
 
-libmeikey_hip.so      hipcc --offload-arch=gfx950: kernels + C ABI (include/meikey_hip.h)
+libmeikey_hip.so      hipcc --offload-arch=gfx950: kernels + C ABI (include/meikey_hip.h), plus the
+                      host re-derivation of emitted keys compiled by g++
 libmeikey_hostcheck.so  g++: the same arithmetic headers compiled for the host, test hooks only
 """
 from __future__ import annotations
@@ -28,9 +29,24 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build failed ({res.returncode}): {' '.join(cmd)}")
 
 
+# Host-only parts of the library, compiled by the host compiler and linked in.
+HOST_SOURCES = [os.path.join(CSRC, "mk_confirm_host.cpp")]
+
+
 def build_hip(out: str = LIB) -> str:
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, *HIP_SOURCES])
+    objs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(CSRC, os.path.basename(src) + ".o")
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-c", "-o", obj, src])
+        objs.append(obj)
+    try:
+        # the host objects go straight to the linker (hipcc would read them as HIP sources)
+        _run([HIPCC, *HIP_FLAGS, "-o", tmp, *HIP_SOURCES, *[a for o in objs for a in ("-Xlinker", o)]])
+    finally:
+        for obj in objs:
+            if os.path.exists(obj):
+                os.remove(obj)
     os.replace(tmp, out)
     return out
 

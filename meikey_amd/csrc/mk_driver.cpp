@@ -16,6 +16,12 @@
 #include "mk_internal.h"
 #include "mk_search.h"
 
+// mk_confirm_host.cpp (host compiler): 1 when the key at `index` re-derives from the search
+// key to `seed` and `pk` and matches the pattern with its x sign, 0 when not, -1 bad pattern
+extern "C" __attribute__((visibility("hidden"))) int mk_confirm_emitted(
+    const uint8_t key[32], int source, uint64_t index, const uint8_t seed[32], const uint8_t pk[32],
+    const uint8_t* needle, uint32_t len, uint32_t fold, uint32_t mode);
+
 namespace {
 
 const char kB64[] = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
@@ -147,6 +153,12 @@ mk_status mk_search(mk_gpu* g, const mk_search_config* cfg, mk_progress_fn progr
         out->index = r.index;
         memcpy(out->seed, eng.first_hit.seed, 32);
         memcpy(out->pk, eng.first_hit.pk, 32);
+        // the reference's confirm: the key is re-derived on the host before it is handed out
+        if (mk_confirm_emitted(key, cfg->seed_source, r.index, out->seed, out->pk, needle, len, fold, mode) != 1) {
+            out->found = 0;
+            return mk_fail(MK_ERR_SELF_CHECK, "the device's key at index %llu does not re-derive on the host",
+                           (unsigned long long)r.index);
+        }
     }
     return MK_OK;
 }

@@ -14,6 +14,7 @@
 #include "mk_hash.h"
 #include "mk_match.h"
 #include "mk_point.h"
+#include "mk_confirm.h"
 #include "mk_tile.h"
 #include "mk_search.h"
 
@@ -152,6 +153,21 @@ int hc_match(const u32 pk[8], const u8* needle, u32 len, u32 fold, u32 mode, u32
     mt.dfa = m.dfa.empty() ? nullptr : m.dfa.data();
     return mk_pk_match(pk, mt, sign_unknown != 0) ? 1 : 0;
 }
+
+// The host confirmation of an emitted key (1 = confirmed, 0 = refused, -1 = bad pattern).
+int hc_confirm(const u32 key[8], u32 source, u64 index, const u8 seed[32], const u8 pk[32], const u8* needle,
+               u32 len, u32 fold, u32 mode) {
+    matcher_host m;
+    if (build_matcher(needle, len, fold, mode, m) != 0) return -1;
+    mk_matcher mt;
+    mt.mode = m.mode;
+    mt.len = m.len;
+    mt.mask = m.mask;
+    mt.dfa = m.dfa.empty() ? nullptr : m.dfa.data();
+    return mk_confirm_key(key, source, index, seed, pk, mt) ? 1 : 0;
+}
+
+void hc_pk_by_ladder(const u32 s[8], u32 pk[8]) { mk_pk_by_ladder(s, pk); }
 
 // search_with over a scripted engine and clock (Python callbacks), as the reference tests
 // its control loop with a mock Engine.

@@ -371,3 +371,25 @@ MK_DEV void mk_table_entry(u32 w, const mk_ext* bases, u32 row, u32 k, u32* out)
     out[30] = 0;
     out[31] = 0;
 }
+
+// s*B by textbook double-and-add over the 255 scalar bits with the unified addition:
+// no table, about 380 additions. Used on the host to re-derive emitted keys
+// independently of the comb table.
+MK_DEV void mk_pk_by_ladder(const u32 s[8], u32 pk[8]) {
+    const fe d2 = mk_const(MK_D2);
+    mk_ext b;
+    b.x = mk_const(MK_BX);
+    b.y = mk_const(MK_BY);
+    b.z = fe_small(1);
+    b.t = fe_mul(b.x, b.y);
+    mk_ext acc = mk_ext_identity();
+    for (int i = 254; i >= 0; i--) {
+        acc = mk_ext_add(acc, acc, d2);
+        if ((s[i >> 5] >> (i & 31)) & 1u) acc = mk_ext_add(acc, b, d2);
+    }
+    const fe zi = fe_invert(acc.z);
+    u32 xw[8];
+    fe_to_words(fe_mul(acc.y, zi), pk);
+    fe_to_words(fe_mul(acc.x, zi), xw);
+    pk[7] |= (xw[0] & 1u) << 31;
+}

@@ -37,6 +37,8 @@ def lib():
         L.hc_lane.argtypes = [P, U32, P, U32, U64, U32, ctypes.c_char_p, U32, U32, U32, P, P]
         L.hc_build_mask.argtypes = [ctypes.c_char_p, U32, U32, U32, P]
         L.hc_match.argtypes = [P, ctypes.c_char_p, U32, U32, U32, U32]
+        L.hc_confirm.argtypes = [P, U32, U64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, U32, U32]
+        L.hc_pk_by_ladder.argtypes = [P, P]
         _lib = L
     return _lib
 
@@ -65,6 +67,18 @@ def secret_scalar_counter(key: bytes, index: int) -> bytes:
     out = (ctypes.c_uint32 * 8)()
     lib().hc_secret_scalar_counter(words(key), index, out)
     return wbytes(out)
+
+
+def pk_by_ladder(s: bytes) -> bytes:
+    out = (ctypes.c_uint32 * 8)()
+    lib().hc_pk_by_ladder(words(s), out)
+    return wbytes(out)
+
+
+def confirm(key: bytes, source: int, index: int, seed_: bytes, pk: bytes, needle: bytes, fold: bool,
+            mode: int) -> int:
+    n = len(needle) // 130 if mode == 2 else len(needle)
+    return lib().hc_confirm(words(key), source, index, seed_, pk, needle, n, int(fold), mode)
 
 
 def limbs_of(x: int):

@@ -186,3 +186,36 @@ def test_bitap_mask_folds_case(hostcheck):
     assert sum(1 for x in m if x) == 4
     m = hostcheck.build_mask(b"aB", False)
     assert m[alpha.index("a")] == 1 and m[alpha.index("A")] == 0 and m[alpha.index("B")] == 2
+
+
+def test_ladder_public_key(oracle, hostcheck):
+    """The table-free double-and-add used to re-derive emitted keys on the host."""
+    rng = random.Random(26)
+    for _ in range(40):
+        seed = rng.randbytes(32)
+        assert hostcheck.pk_by_ladder(oracle.secret_scalar(seed)) == oracle.public_key(seed)
+
+
+def test_host_confirm_accepts_true_keys_and_refuses_others(oracle, hostcheck):
+    """The confirmation before a key is handed out (the reference's confirm,
+    gpu.rs:973-996, whose tests require a wrong hit to be refused): the true key passes;
+    a flipped x sign, another index's seed or key, or a pattern it does not match fail."""
+    import meikey_amd as M
+    rng = random.Random(27)
+    for source in (0, 1):
+        key = rng.randbytes(32)
+        for _ in range(6):
+            i = rng.getrandbits(48)
+            seed, pk = oracle.seed(key, source, i), oracle.derive(key, source, i, 1)
+            b64 = oracle.public_b64(pk)
+            needle = b64[60:66].encode()
+            assert hostcheck.confirm(key, source, i, seed, pk, needle, False, 0) == 1
+            assert hostcheck.confirm(key, source, i, seed, pk, b64[62:].encode(), False, 1) == 1
+            dfa = M.Pattern(b64[40:44], True, M.MODE_REGEX).needle()[0]
+            assert hostcheck.confirm(key, source, i, seed, pk, dfa, False, 2) == 1
+            flipped = pk[:31] + bytes([pk[31] ^ 0x80])
+            assert hostcheck.confirm(key, source, i, seed, flipped, needle, False, 0) == 0
+            other = oracle.derive(key, source, i + 1, 1)
+            assert hostcheck.confirm(key, source, i, seed, other, needle, False, 0) == 0
+            assert hostcheck.confirm(key, source, i, oracle.seed(key, source, i + 1), pk, needle, False, 0) == 0
+            assert hostcheck.confirm(key, source, i, seed, pk, b"+/+/+/", False, 0) == 0

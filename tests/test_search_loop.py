@@ -111,3 +111,46 @@ def test_engine_errors_propagate():
     s = Script(kpi=10, fail_at=3)
     r = s.run()
     assert r["status"] == 5 and len(s.calls) == 3
+
+
+def test_search_with_on_a_host_engine(oracle):
+    """The reference also runs search_with over a host-compiled engine (gpu.rs:2935-3215).
+    Here each launch computes its keys with the kernel's own code on the CPU: the lane
+    tile (comb W = 8) flags sign-unknown candidates, the full key with its x sign comes
+    from the host-compiled comb, and the host confirmation accepts a hit. The controller
+    must return the oracle's first hit of the index sequence."""
+    import time
+    w, kpi = 8, 64
+    tab = H.table(w)
+    key = bytes(range(7, 39))
+    needle, fold, mode = b"k9", True, 0
+    calls = []
+
+    def launch(_, base, iters, found, index):
+        calls.append((base, iters))
+        for it in range(iters):
+            first = base + it * kpi
+            _, cand = H.lane(tab, w, key, 0, first, kpi, needle, fold, mode)
+            for j in range(kpi):
+                if not cand[j]:
+                    continue
+                i = first + j
+                seed = H.seed(key, 0, i)
+                pk = H.full_pk(tab, w, H.secret_scalar(seed))
+                if H.confirm(key, 0, i, seed, pk, needle, fold, mode) == 1:
+                    found[0], index[0] = 1, i
+                    return 0
+        return 0
+
+    t0 = time.perf_counter()
+    out = (ctypes.c_uint64 * 5)()
+    cbs = (LAUNCH(launch), CLOCK(lambda _: time.perf_counter() - t0), REPORT(lambda *a: None), CANCEL(lambda _: 0))
+    L = H.lib()
+    L.hc_search_with.argtypes = [LAUNCH, CLOCK, REPORT, CANCEL, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+    L.hc_search_with(*cbs, None, kpi, 0, 0.2, 0.0, 5.0, out)
+    status, found, index = out[0], out[1], out[2]
+    assert status == 0 and found == 1
+    assert [b for b, _ in calls] == sorted(b for b, _ in calls) and calls[0][0] == 0
+    want = oracle.search(key, 0, 0, index + 1, needle, not fold, mode)
+    assert want and want[0] == index
