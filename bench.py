@@ -114,7 +114,11 @@ def main():
                      "kernel": "mk_search_kernel", "kernel_ms": avg_ms,
                      "algorithmic": f"{mults:.2f} field mults/key (W=16, K=256) x 64 32x32-bit products",
                      "achieved_run_mults": achieved_run / 1e12,
-                     "run_mults_per_key": run_mults},
+                     "run_mults_per_key": run_mults,
+                     # integer VALU is the bound (no dense contraction for MFMA); the HBM
+                     # side of the same launches, from the committed traffic counters
+                     "hbm": ({"achieved": traffic / (avg_ms * 1e-3) / 1e12, "peak": 8.0, "unit": "TB/s",
+                              "frac": traffic / (avg_ms * 1e-3) / 8e12} if traffic else None)},
     }
     if not args.no_batch:
         out["batched"] = batched_leg(gpu, rate)
