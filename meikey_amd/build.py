@@ -3,6 +3,7 @@
 libmeikey_hip.so      hipcc --offload-arch=gfx950: kernels + C ABI (include/meikey_hip.h), plus the
                       host re-derivation of emitted keys compiled by g++
 libmeikey_hostcheck.so  g++: the same arithmetic headers compiled for the host, test hooks only
+mk_example            gcc: a C client of the ABI (open, pattern, search, OpenSSH line)
 """
 from __future__ import annotations
 
@@ -14,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmeikey_hip.so")
 HOSTCHECK = os.path.join(HERE, "libmeikey_hostcheck.so")
+EXAMPLE = os.path.join(HERE, "mk_example")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # gfx950 only; packed-FP32 instructions stay off (the kernels are integer code anyway).
@@ -59,9 +61,18 @@ def build_hostcheck(out: str = HOSTCHECK) -> str:
     return out
 
 
+def build_example(out: str = EXAMPLE) -> str:
+    """The C client of the ABI (csrc/mk_example.c), linked against the in-tree library."""
+    include = os.path.join(os.path.dirname(HERE), "include")
+    _run(["gcc", "-O2", "-std=c11", "-Wall", "-I", include, "-o", out, os.path.join(CSRC, "mk_example.c"),
+          "-L", HERE, "-lmeikey_hip", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def build_all() -> None:
     build_hostcheck()
     build_hip()
+    build_example()
 
 
 if __name__ == "__main__":

@@ -546,6 +546,26 @@ def test_searcher_exact_hit_and_openssh(gpu, oracle, tmp_path):
         assert pub.endswith(pk) and seed + pk in priv
 
 
+def test_c_client_of_the_abi(oracle):
+    """meikey_amd/mk_example (csrc/mk_example.c, plain C against include/meikey_hip.h):
+    its Searcher run returns the oracle's first hit and the matching OpenSSH line."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(HERE), "meikey_amd", "mk_example")
+    if not os.path.exists(exe):
+        pytest.skip("C client not built")
+    key = random.Random(99).randbytes(32)
+    say("run mk_example")
+    res = subprocess.run([exe, "Zq", key.hex()], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    lines = res.stdout.splitlines()
+    index = int(lines[0].split()[1])
+    pk = bytes.fromhex(lines[1].split()[1])
+    assert pk == oracle.derive(key, 0, index, 1)
+    first = oracle.search(key, 0, 0, index + 1, b"zq", False, M.MODE_SUFFIX)
+    assert first and first[0] == index
+    assert lines[3] == M.openssh_public(pk, "example")
+
+
 def test_searcher_bench_runs_to_the_time_limit(gpu_small):
     pat = M.Pattern("A", True, M.MODE_CONTAINS)
     out = gpu_small.search(pat, M.SEED_COUNTER, bench=True, time_limit=0.5, target_launch=0.05)
