@@ -5,11 +5,13 @@
 
 #include <algorithm>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "mk_hitsort.h"
 #include "mk_internal.h"
 
 namespace {
@@ -106,26 +108,21 @@ mk_status mk_run_step(mk_gpu* g, const mk_search_desc* descs, const u64* masks, 
 
 namespace {
 
-// The n exact hits as caller records, ordered by (search, index). The order is
-// sorted on 16-byte keys and the 80-byte records are gathered once.
-void to_records(const mk_hit* hs, u32 n, mk_hit_rec* recs, u32 cap) {
-    struct key {
-        u64 index;
-        u32 search, pos;
-    };
-    std::vector<key> order(n);
-    for (u32 i = 0; i < n; i++) order[i] = {hs[i].index, hs[i].search, i};
-    std::sort(order.begin(), order.end(), [](const key& a, const key& b) {
-        return a.search != b.search ? a.search < b.search : a.index < b.index;
-    });
-    const size_t m = std::min<size_t>(n, cap);
-    for (size_t i = 0; i < m; i++) {
-        const mk_hit& h = hs[order[i].pos];
-        recs[i].index = h.index;
-        recs[i].search = h.search;
+// The n exact hits as caller records, ordered by (search, index) (mk_hitsort.h). mk_hit
+// has the record's layout, so the gather is one copy per hit.
+static_assert(sizeof(mk_hit) == sizeof(mk_hit_rec) && offsetof(mk_hit, index) == offsetof(mk_hit_rec, index) &&
+                  offsetof(mk_hit, search) == offsetof(mk_hit_rec, search) &&
+                  offsetof(mk_hit, seed) == offsetof(mk_hit_rec, seed) && offsetof(mk_hit, pk) == offsetof(mk_hit_rec, pk),
+              "mk_hit must have the layout of mk_hit_rec");
+static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "hit words are copied as little-endian bytes");
+
+void to_records(mk_gpu* g, const mk_hit* hs, u32 n, mk_hit_rec* recs, u32 cap) {
+    const u32* order = mk_sort_hits(hs, n, g->sort);
+    const u32 m = std::min(n, cap);
+    for (u32 i = 0; i < m; i++) {
+        if (i + 8 < m) __builtin_prefetch(&hs[order[i + 8]]);
+        memcpy(&recs[i], &hs[order[i]], sizeof(mk_hit_rec));
         recs[i].pad = 0;
-        mk_words_to_bytes(h.seed, recs[i].seed);
-        mk_words_to_bytes(h.pk, recs[i].pk);
     }
 }
 
@@ -415,7 +412,7 @@ mk_status mk_launch_run(mk_launch* l, uint64_t base, uint32_t iters, mk_hit_rec*
                                overflow);
     if (st != MK_OK) return st;
     *n_hits = nh;
-    to_records(hs, std::min(nh, kHitCap), hits, cap);
+    to_records(g, hs, std::min(nh, kHitCap), hits, cap);
     // a stop-on-hit search needs one hit, not the full set: lists that overflowed while
     // the tiles already in flight finished are not an error
     if ((overflow || nh > cap) && !l->stop_on_hit)
@@ -554,7 +551,7 @@ mk_status mk_batch_run(mk_gpu* g, const mk_search_spec* specs, uint32_t n, mk_hi
                      n_cands, nh, overflow);
     if (st != MK_OK) return st;
     *n_hits = nh;
-    to_records(hs, std::min(nh, kHitCap), hits, cap);
+    to_records(g, hs, std::min(nh, kHitCap), hits, cap);
     bool all_stop = true;
     for (u32 i = 0; i < n; i++) all_stop = all_stop && specs[i].stop_on_hit;
     if ((overflow || nh > cap) && !all_stop)

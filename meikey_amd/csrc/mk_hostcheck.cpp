@@ -4,7 +4,7 @@
 // exact code the gfx950 kernels run; compiled here with g++ they expose test hooks so
 // the CPU test suite can compare every stage (seeds, scalar, field operations at their
 // limb bounds, digits, table entries, the search lane's batched inversion, matching)
-// with the oracle. This library is test infrastructure: nothing in the product loads it.
+// with the oracle, and the host-side hit order (mk_hitsort.h). This library is test infrastructure: nothing in the product loads it.
 #include <stdio.h>
 #include <string.h>
 
@@ -17,6 +17,7 @@
 #include "mk_confirm.h"
 #include "mk_tile.h"
 #include "mk_search.h"
+#include "mk_hitsort.h"
 
 #define MK_MATCHER_FAIL(st, ...) (fprintf(stderr, __VA_ARGS__), fputc('\n', stderr), (st))
 #include "mk_matcher_host.h"
@@ -168,6 +169,19 @@ int hc_confirm(const u32 key[8], u32 source, u64 index, const u8 seed[32], const
 }
 
 void hc_pk_by_ladder(const u32 s[8], u32 pk[8]) { mk_pk_by_ladder(s, pk); }
+
+// The library's hit order over n (search, index) pairs: order[i] = position of the i-th.
+void hc_sort_hits(const u32* search, const u64* index, u32 n, u32* order) {
+    struct hit {
+        u64 index;
+        u32 search;
+    };
+    std::vector<hit> hs(n);
+    for (u32 i = 0; i < n; i++) hs[i] = {index[i], search[i]};
+    static mk_sort_pool pool;  // reused across calls, as the library reuses its own
+    const u32* o = mk_sort_hits(hs.data(), n, pool);
+    for (u32 i = 0; i < n; i++) order[i] = o[i];
+}
 
 // search_with over a scripted engine and clock (Python callbacks), as the reference tests
 // its control loop with a mock Engine.

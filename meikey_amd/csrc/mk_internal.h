@@ -7,13 +7,14 @@
 #include <vector>
 
 #include "../../include/meikey_hip.h"
+#include "mk_hitsort.h"
 #include "mk_launch.h"
 
 // Records the text of a failure for mk_last_error() and returns st.
 mk_status mk_fail(mk_status st, const char* fmt, ...);
 
 // One step's fixed device lists: candidates of the search kernel (16 B each) and exact
-// hits of the confirm kernel (96 B each).
+// hits of the confirm kernel (80 B each).
 constexpr u32 kCandCap = 1u << 20;
 constexpr u32 kHitCap = 1u << 20;
 
@@ -61,6 +62,7 @@ struct mk_gpu {
     // per-call pools of the batched path
     dbuf descs, masks, dfas, segs, stop;
     std::vector<u8> staging;
+    mk_sort_pool sort;  // host buffers of the hit sort, kept between calls
     // events around the last step's search and confirm kernels (on `stream`)
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float last_search_ms = 0.f, last_confirm_ms = 0.f;

@@ -32,11 +32,13 @@ struct mk_cand {
     u64 index;
 };
 
-// An exact hit: the key matches with its real x sign.
+// An exact hit: the key matches with its real x sign. The fields are in the order of
+// the caller's record (mk_hit_rec: index, search, pad, seed bytes, pk bytes), and the
+// words are little-endian, so on the host a hit converts to a record by one copy.
 struct mk_hit {
+    u64 index;
     u32 search;
     u32 pad;
-    u64 index;
     u32 seed[8];
     u32 pk[8];
 };

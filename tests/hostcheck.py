@@ -39,6 +39,7 @@ def lib():
         L.hc_match.argtypes = [P, ctypes.c_char_p, U32, U32, U32, U32]
         L.hc_confirm.argtypes = [P, U32, U64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, U32, U32]
         L.hc_pk_by_ladder.argtypes = [P, P]
+        L.hc_sort_hits.argtypes = [P, P, U32, P]
         _lib = L
     return _lib
 
@@ -177,3 +178,13 @@ def match(pk: bytes, needle: bytes, fold: bool, mode: int, sign_unknown: bool) -
     rc = lib().hc_match(words(pk), needle, n, int(fold), mode, int(sign_unknown))
     assert rc >= 0
     return bool(rc)
+
+
+def sort_hits(search, index):
+    """The library's (search, index) order of the hits: a list of positions."""
+    import numpy as np
+    s = np.ascontiguousarray(search, dtype=np.uint32)
+    x = np.ascontiguousarray(index, dtype=np.uint64)
+    out = np.zeros(len(s), dtype=np.uint32)
+    lib().hc_sort_hits(s.ctypes.data, x.ctypes.data, len(s), out.ctypes.data)
+    return out

@@ -220,3 +220,44 @@ def test_host_confirm_accepts_true_keys_and_refuses_others(oracle, hostcheck):
             assert hostcheck.confirm(key, source, i, seed, other, needle, False, 0) == 0
             assert hostcheck.confirm(key, source, i, oracle.seed(key, source, i + 1), pk, needle, False, 0) == 0
             assert hostcheck.confirm(key, source, i, seed, pk, b"+/+/+/", False, 0) == 0
+
+
+@pytest.mark.parametrize("case", ["empty", "one", "dense", "wide", "sparse_searches", "duplicates"])
+def test_hit_order(hostcheck, case):
+    """The library's hit order (counting sort by search, radix sort by index offset) is a
+    permutation that puts the hits in (search, index) order, as numpy's lexsort does,
+    for index ranges that need 0 to 6 radix passes."""
+    import numpy as np
+    rng = np.random.default_rng(zlib.crc32(case.encode()))
+    if case == "empty":
+        s, x = np.zeros(0, np.uint32), np.zeros(0, np.uint64)
+    elif case == "one":
+        s, x = np.array([7], np.uint32), np.array([2**63 + 5], np.uint64)
+    elif case == "dense":  # 64 searches of 2^26 keys, as in a batched launch
+        s = rng.integers(0, 64, 200000).astype(np.uint32)
+        x = (rng.choice(1 << 26, 200000, replace=False).astype(np.uint64) + (s.astype(np.uint64) << np.uint64(40)))
+    elif case == "wide":  # ranges of 2^12 .. 2^64 within a search
+        s = rng.integers(0, 6, 30000).astype(np.uint32)
+        span = np.array([1 << 12, 1 << 22, 1 << 33, 1 << 44, 1 << 55, 2**64 - 1], dtype=object)
+        x = np.array([int(rng.integers(0, 2**63)) * 2 % int(span[k]) if span[k] > 1 else 0 for k in s], np.uint64)
+        x[:6] = [0, 2**64 - 1, 1, 2**64 - 2, 2**32, 2**63]
+        s[:6] = 5
+    elif case == "sparse_searches":  # empty and one-hit groups between large ones
+        s = rng.choice([0, 1, 3, 900, 4095], 5000).astype(np.uint32)
+        s[0] = 2
+        x = rng.integers(0, 2**40, 5000).astype(np.uint64)
+    else:  # equal keys: any order among them, the same sorted keys
+        s = rng.integers(0, 3, 3000).astype(np.uint32)
+        x = rng.integers(0, 50, 3000).astype(np.uint64)
+    order = hostcheck.sort_hits(s, x)
+    assert sorted(order.tolist()) == list(range(len(s)))
+    want = np.lexsort((x, s))
+    assert s[order].tolist() == s[want].tolist()
+    assert x[order].tolist() == x[want].tolist()
+    # a second call on a shorter, reordered input reuses the buffers without carrying
+    # anything over
+    s2, x2 = s[::-2].copy(), x[::-2].copy()
+    order2 = hostcheck.sort_hits(s2, x2)
+    assert sorted(order2.tolist()) == list(range(len(s2)))
+    want2 = np.lexsort((x2, s2))
+    assert s2[order2].tolist() == s2[want2].tolist() and x2[order2].tolist() == x2[want2].tolist()
