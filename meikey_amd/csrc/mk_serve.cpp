@@ -211,6 +211,7 @@ mk_status admit(mk_server* s) {
             MK_HIPS(hipMemcpy(s->d_dfas + off, m.dfa.data(), m.dfa.size() * sizeof(u16), hipMemcpyHostToDevice));
         const u64 first = p.spec.first;
         MK_HIPS(hipMemcpy(s->d_next + slot, &first, sizeof first, hipMemcpyHostToDevice));
+        s->h_next[slot] = first;  // the host copy is refreshed by every step; start it right
         s->queue.pop_front();
         (void)g;
     }
@@ -342,7 +343,10 @@ mk_status mk_server_step(mk_server* s, uint32_t* active_out, uint32_t* queued_ou
     if (nh) {
         MK_HIPS(hipMemcpy(hits.data(), g->hits, sizeof(mk_hit) * nh, hipMemcpyDeviceToHost));
     }
-    // lowest-index exact hit of each slot in this step
+    // lowest-index exact hit of each slot in this step. A full candidate list (overflow)
+    // drops candidates but not their slots' stop flags, so such a slot may finish on a
+    // later step's hit: every result is still an exact, confirmed match, only not always
+    // the lowest matching index of the range searched.
     std::map<u32, mk_hit> best;
     for (const mk_hit& h : hits) {
         if (h.search >= s->opt.max_slots || !s->slots[h.search].used) continue;
