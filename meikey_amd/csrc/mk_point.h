@@ -53,11 +53,15 @@ MK_DEV void mk_add_niels_frac(const mk_ext& p, const mk_niels& q, fe& yn, fe& yd
     xd = fe_carry(fe_add(z2, txy));
 }
 
-// Row 0's entry (third field 2xy) to extended coordinates with no multiplication.
+// Row 0's entry (third field 2xy) to extended coordinates with no multiplication and no
+// carry pass: X = ypx + 2p - ymx = 2x, Y = ypx + ymx = 2y, Z = 2, T = 2xy. The point only
+// ever enters mk_add_niels as p, where Y + X = 2 ypx + 2p (limbs < 2^28) and
+// Y - X = Y + 2p - X = 2 ymx (exact limb by limb: no limb borrows, although X's limbs
+// exceed 2p's) are both first operands of fe_mul, as that addition requires.
 MK_DEV mk_ext mk_row0_extended(const mk_niels& n) {
     mk_ext p;
-    p.x = fe_carry(fe_sub(n.ypx, n.ymx));
-    p.y = fe_carry(fe_add(n.ypx, n.ymx));
+    p.x = fe_sub(n.ypx, n.ymx);
+    p.y = fe_add(n.ypx, n.ymx);
     p.z = fe_small(2);
     p.t = n.xy2d;
     return p;
