@@ -156,10 +156,13 @@ MK_DEV fe fe_invert(const fe& a) {
     return fe_mul(fe_pow2k(a_250_0, 5), a11);
 }
 
-// Canonical value in [0, p) as eight little-endian 32-bit words.
-MK_DEV void fe_to_words(const fe& a, u32 w[8]) {
-    fe h = fe_carry(a);
-    // q = floor((h + 19) / 2^255) is 1 exactly when h >= p
+// Canonical value in [0, p) as eight little-endian 32-bit words, of h whose value is
+// below 2p: what fe_mul, fe_sq and fe_carry return (their value is below
+// 2^255 + 2^142; limb 1 below 2^25 + 2^18.3, limb 5 below 2^25 + 2^14, the others
+// within their radix).
+MK_DEV void fe_carried_to_words(const fe& h, u32 w[8]) {
+    // q = floor((h + 19) / 2^255) is 1 exactly when h >= p; the chain of floor
+    // divisions is exact for any limbs, and h < 2p keeps q at 0 or 1
     u32 q = (h.v[0] + 19u) >> 26;
     q = (h.v[1] + q) >> 25;
     q = (h.v[2] + q) >> 26;
@@ -190,6 +193,9 @@ MK_DEV void fe_to_words(const fe& a, u32 w[8]) {
     w[6] = v[7] >> 13 | v[8] << 12;
     w[7] = v[8] >> 20 | v[9] << 6;
 }
+
+// Canonical value of any element (limbs below 2^31) as eight little-endian words.
+MK_DEV void fe_to_words(const fe& a, u32 w[8]) { fe_carried_to_words(fe_carry(a), w); }
 
 // Any 256-bit little-endian value (bit 255 folded as 19) to carried limbs.
 MK_DEV fe fe_from_words(const u32 w[8]) {

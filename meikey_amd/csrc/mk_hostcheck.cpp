@@ -60,6 +60,12 @@ void hc_fe_to_words(const u32 f[10], u32 out[8]) {
     fe_to_words(a, out);
 }
 
+void hc_fe_carried_to_words(const u32 f[10], u32 out[8]) {
+    fe a;
+    memcpy(a.v, f, 40);
+    fe_carried_to_words(a, out);
+}
+
 void hc_fe_from_words(const u32 w[8], u32 out[10]) {
     const fe a = fe_from_words(w);
     memcpy(out, a.v, 40);

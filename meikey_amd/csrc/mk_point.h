@@ -21,7 +21,8 @@ struct mk_niels {
 };
 
 // Extended + affine Niels point, 7 multiplications (madd for a = -1 with the
-// 2d factor folded into the table). Inputs: p carried, q carried.
+// 2d factor folded into the table). Inputs: p carried (or row 0's point from
+// mk_row0_extended), q's fields carried or negated.
 MK_DEV mk_ext mk_add_niels(const mk_ext& p, const mk_niels& q) {
     const fe pp = fe_mul(fe_add(p.y, p.x), q.ypx);
     const fe mm = fe_mul(fe_sub(p.y, p.x), q.ymx);
@@ -41,14 +42,16 @@ MK_DEV mk_ext mk_add_niels(const mk_ext& p, const mk_niels& q) {
 
 // The last addition as fractions: y = (pp + mm) / (2Z - T*2dxy) and
 // x = (pp - mm) / (2Z + T*2dxy), 3 multiplications (reference kernel.cu:1613-1627).
-// Denominators are returned carried.
+// yd is left uncarried: it is mk_add_niels's t (limbs below 2^28 / 2^27 + 2^19.3), so
+// callers take it only as the first operand of fe_mul. xd is carried; yn and xn are
+// sums and differences of products (second-operand bounds).
 MK_DEV void mk_add_niels_frac(const mk_ext& p, const mk_niels& q, fe& yn, fe& yd, fe& xn, fe& xd) {
     const fe pp = fe_mul(fe_add(p.y, p.x), q.ypx);
     const fe mm = fe_mul(fe_sub(p.y, p.x), q.ymx);
     const fe txy = fe_mul(p.t, q.xy2d);
     const fe z2 = fe_add(p.z, p.z);
     yn = fe_add(pp, mm);
-    yd = fe_carry(fe_sub(z2, txy));
+    yd = fe_sub(z2, txy);
     xn = fe_sub(pp, mm);
     xd = fe_carry(fe_add(z2, txy));
 }
@@ -274,10 +277,10 @@ MK_DEV void mk_full_pk(const u32* __restrict__ table, u32 w, const u32 s[8], u32
     mk_mul_base_frac(table, w, s, yn, yd, xn, xd);
     const fe inv = fe_invert(fe_mul(yd, xd));
     const fe y = fe_mul(fe_mul(yn, xd), inv);
-    const fe x = fe_mul(fe_mul(xn, yd), inv);
+    const fe x = fe_mul(fe_mul(yd, xn), inv);
     u32 xw[8];
-    fe_to_words(y, pk);
-    fe_to_words(x, xw);
+    fe_carried_to_words(y, pk);
+    fe_carried_to_words(x, xw);
     pk[7] |= (xw[0] & 1u) << 31;
 }
 
@@ -393,7 +396,7 @@ MK_DEV void mk_pk_by_ladder(const u32 s[8], u32 pk[8]) {
     }
     const fe zi = fe_invert(acc.z);
     u32 xw[8];
-    fe_to_words(fe_mul(acc.y, zi), pk);
-    fe_to_words(fe_mul(acc.x, zi), xw);
+    fe_carried_to_words(fe_mul(acc.y, zi), pk);
+    fe_carried_to_words(fe_mul(acc.x, zi), xw);
     pk[7] |= (xw[0] & 1u) << 31;
 }

@@ -37,7 +37,7 @@ template <class M, class P, class F, class G>
 MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const mk_spill& sp, M mul, P pre_sweep,
                          F on_key, G mid) {
     u32 upper[8];
-    fe pre;
+    fe pre = fe_small(1);  // prefix product of the yd (each yd is a first operand of fe_mul)
     const u64 tail = (u64)key[7] << 32 | key[6];
     MK_NO_UNROLL
     for (u32 j = 0; j < K; j++) {
@@ -52,7 +52,7 @@ MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const m
         }
         fe yn, yd;
         mul(s, yn, yd);
-        pre = j == 0 ? yd : fe_mul(pre, yd);
+        pre = fe_mul(yd, pre);
         sp.store(3 * j, yn);
         sp.store(3 * j + 1, yd);
         sp.store(3 * j + 2, pre);
@@ -80,10 +80,10 @@ MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const m
         fe zinv = inv;
         if (jj > 0) {
             zinv = fe_mul(inv, pre_prev);
-            inv = fe_mul(inv, yd);
+            inv = fe_mul(yd, inv);
         }
         u32 pk[8];
-        fe_to_words(fe_mul(yn, zinv), pk);
+        fe_carried_to_words(fe_mul(yn, zinv), pk);
         on_key(jj, pk);
     }
 }

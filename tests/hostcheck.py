@@ -40,6 +40,7 @@ def lib():
         L.hc_confirm.argtypes = [P, U32, U64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, U32, U32]
         L.hc_pk_by_ladder.argtypes = [P, P]
         L.hc_sort_hits.argtypes = [P, P, U32, P]
+        L.hc_fe_carried_to_words.argtypes = [P, P]
         _lib = L
     return _lib
 
@@ -188,3 +189,11 @@ def sort_hits(search, index):
     out = np.zeros(len(s), dtype=np.uint32)
     lib().hc_sort_hits(s.ctypes.data, x.ctypes.data, len(s), out.ctypes.data)
     return out
+
+
+def fe_carried_to_int(limbs) -> int:
+    """fe_carried_to_words of carried limbs, as an integer."""
+    a = (ctypes.c_uint32 * 10)(*limbs)
+    out = (ctypes.c_uint32 * 8)()
+    lib().hc_fe_carried_to_words(a, out)
+    return int.from_bytes(struct.pack("<8I", *out), "little")

@@ -58,9 +58,13 @@ def test_fe_mul_within_bounds(hostcheck, case):
             g = [(3 << 26) + (3 << 13) if i % 2 == 0 else (3 << 25) + (3 << 13) for i in range(10)]
         r = hostcheck.fe_mul_limbs(f, g)
         assert all(x < (1 << 26) + (1 << 14) for x in r)
-        assert hostcheck.fe_to_int(r) == hostcheck.value_of(f) * hostcheck.value_of(g) % P
+        want = hostcheck.value_of(f) * hostcheck.value_of(g) % P
+        assert hostcheck.fe_to_int(r) == want
+        assert hostcheck.value_of(r) < 2**255 + 2**142  # the bound fe_carried_to_words relies on
+        assert hostcheck.fe_carried_to_int(r) == want
         q = hostcheck.fe_sq_limbs(r)
         assert hostcheck.fe_to_int(q) == hostcheck.value_of(r) ** 2 % P
+        assert hostcheck.fe_carried_to_int(q) == hostcheck.value_of(r) ** 2 % P
 
 
 def test_fe_ops_edge_values(oracle, hostcheck):
@@ -81,6 +85,29 @@ def test_to_words_canonical(hostcheck):
     for v in [P - 1, P, P + 1, P + 18, 2**255 - 1, 0, 1]:
         limbs = hostcheck.limbs_of(v % (1 << 255))
         assert hostcheck.fe_to_int(limbs) == v % P
+        assert hostcheck.fe_carried_to_int(limbs) == v % P
+
+
+def test_carried_to_words_at_the_carried_bound(hostcheck):
+    """fe_carried_to_words takes what fe_mul returns without a carry pass: limb 1 up to
+    2^25 + 2^18.3 and limb 5 up to 2^25 + 2^14 (values up to 2^255 + 2^142, above p)
+    reduce to the canonical value."""
+    top = [(1 << 26) - 1 if i % 2 == 0 else (1 << 25) - 1 for i in range(10)]
+    cases = [top]
+    for l1 in (1 << 25, (1 << 25) + (1 << 18), (1 << 25) + 321000):
+        for l5 in ((1 << 25) - 1, (1 << 25) + (1 << 14) - 1):
+            cases.append(top[:1] + [l1] + top[2:5] + [l5] + top[6:])
+    rng = random.Random(25)
+    for _ in range(300):
+        c = _rand_limbs(rng, MASK26, MASK25)
+        c[1] = rng.randrange(0, (1 << 25) + 321000)
+        c[5] = rng.randrange(0, (1 << 25) + (1 << 14))
+        cases.append(c)
+    for c in cases:
+        v = hostcheck.value_of(c)
+        assert v < 2 * P
+        assert hostcheck.fe_carried_to_int(c) == v % P, c
+    assert any(hostcheck.value_of(c) >= P for c in cases)
 
 
 def test_digits(oracle, hostcheck):
