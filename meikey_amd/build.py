@@ -19,8 +19,12 @@ EXAMPLE = os.path.join(HERE, "mk_example")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # gfx950 only; packed-FP32 instructions stay off (the kernels are integer code anyway).
+# The ILP-first machine scheduler orders the search kernel's multiply-add columns so that
+# fewer issue slots go idle: 0.1-0.55% faster than the default schedule, four A/B pairs
+# on two boxes (same VGPR count, no scratch).
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-mllvm", "-amdgpu-sched-strategy=max-ilp"]
 HIP_SOURCES = [os.path.join(CSRC, "mk_kernels.hip"), os.path.join(CSRC, "mk_host.cpp"), os.path.join(CSRC, "mk_pattern.cpp"), os.path.join(CSRC, "mk_serve.cpp"), os.path.join(CSRC, "mk_driver.cpp")]
 
 
