@@ -154,8 +154,10 @@ typedef struct {
     uint64_t count; /* number of indices; rounded up to whole tiles of 64 * batch keys */
 } mk_search_spec;
 
-/* Run all `n` searches to completion in one launch. tries (optional, n entries) receives
- * the number of indices each search covered. */
+/* Run all `n` searches to completion in one launch. The exact hits come back ordered by
+ * (search, index); *n_hits and MK_ERR_OVERFLOW as in mk_launch_run (no error when every
+ * search stops on its first hit). tries (optional, n entries) receives the number of
+ * indices each search covered. */
 mk_status mk_batch_run(mk_gpu* g, const mk_search_spec* specs, uint32_t n, mk_hit_rec* hits, uint32_t cap,
                        uint32_t* n_hits, uint64_t* tries);
 
