@@ -133,8 +133,9 @@ mk_status mk_derive(mk_gpu* g, const uint8_t key[32], int seed_source, uint64_t 
 mk_status mk_derive_seeds(mk_gpu* g, const uint8_t* seeds, uint32_t n, uint8_t* pks);
 
 /* Raw comb-table entries (row, k) as stored on the device: 32 words each, three 10-limb
- * field elements (y+x, y-x, third): limbs 0-7 of each in words 0-7 / 8-15 / 16-23,
- * limbs 8-9 in words 24-25 / 26-27 / 28-29, words 30-31 zero. For tests of the table build. */
+ * field elements, the reference's (y+x, y-x, 2dxy) halved: (y+x)/2, (y-x)/2, dxy (row 0:
+ * xy). Limbs 0-7 of each in words 0-7 / 8-15 / 16-23, limbs 8-9 in words 24-25 / 26-27 /
+ * 28-29, words 30-31 zero. For tests of the table build. */
 mk_status mk_table_read(mk_gpu* g, uint32_t row, uint32_t k, uint32_t count, uint32_t* words);
 
 /* ------------------------------------------------------------------------------------

@@ -48,8 +48,8 @@ typedef uint16_t u16;
 MK_DEV u32 mk_rows(u32 w) { return (MK_M_BITS + w - 1) / w; }
 MK_DEV u32 mk_cols(u32 w) { return (1u << (w - 1)) + 1; }
 
-// One comb-table entry: the affine point (y+x, y-x, 2dxy) as three 10-limb field
-// elements (30 words) plus 2 pad words, so an entry is one 128-byte cache line.
+// One comb-table entry: the affine point as ((y+x)/2, (y-x)/2, dxy) (row 0: xy), three
+// 10-limb field elements (30 words) plus 2 pad words, so an entry is one 128-byte cache line.
 #define MK_ENTRY_WORDS 32
 
 // The 44 base64 characters that depend on the key (blob bytes 18..50), and the

@@ -3,11 +3,16 @@
 //
 // The clamped scalar is s = 8 * (2^251 + m). The comb table of width w has
 // rows = ceil(251 / w) rows; entry (i, k) is k * 2^(w*i) * 8B as an affine point in
-// "Niels" form (y+x, y-x, 2dxy), and the top row also adds 2^254 * B, so that
+// "Niels" form, and the top row also adds 2^254 * B, so that
 // s*B = sum_i d_i * entry(i, |d_i|) for the signed base-2^w digits d_i of m
-// (reference src/service/derive/comb.rs:1-8, 289-339, 414-445). Row 0 stores 2xy in
-// place of 2dxy so the first point becomes extended coordinates with no multiply
-// (X = 2x, Y = 2y, Z = 2, T = 2xy; reference src/service/gpu.rs:1466-1470).
+// (reference src/service/derive/comb.rs:1-8, 289-339, 414-445).
+//
+// The reference stores (y+x, y-x, 2dxy), with 2xy in row 0 (src/service/gpu.rs:1455-1480).
+// This table stores every field halved: ((y+x)/2, (y-x)/2, dxy), row 0 with xy. The mixed
+// addition then takes Z where the textbook formula takes 2Z: its A, B, C and D come out
+// halved together and the result is the textbook one divided by 4, the same projective
+// point, without the doubling of Z. Row 0 becomes extended coordinates with no multiply
+// (X = x, Y = y, Z = 1, T = xy).
 #pragma once
 
 #include "mk_field.h"
@@ -16,20 +21,21 @@ struct mk_ext {
     fe x, y, z, t;
 };
 
+// A table entry's fields: (y+x)/2, (y-x)/2 and dxy (row 0: xy).
 struct mk_niels {
-    fe ypx, ymx, xy2d;
+    fe ypx, ymx, xyd;
 };
 
-// Extended + affine Niels point, 7 multiplications (madd for a = -1 with the
-// 2d factor folded into the table). Inputs: p carried (or row 0's point from
+// Extended + halved affine Niels point, 7 multiplications (madd for a = -1 with the
+// d factor folded into the table). pp, mm and txy are the textbook B, A and C halved,
+// and Z stands for D = 2Z halved. Inputs: p carried (or row 0's point from
 // mk_row0_extended), q's fields carried or negated.
 MK_DEV mk_ext mk_add_niels(const mk_ext& p, const mk_niels& q) {
     const fe pp = fe_mul(fe_add(p.y, p.x), q.ypx);
     const fe mm = fe_mul(fe_sub(p.y, p.x), q.ymx);
-    const fe txy = fe_mul(p.t, q.xy2d);
-    const fe z2 = fe_add(p.z, p.z);
+    const fe txy = fe_mul(p.t, q.xyd);
     const fe x = fe_sub(pp, mm), y = fe_add(pp, mm);
-    const fe z = fe_add(z2, txy), t = fe_sub(z2, txy);
+    const fe z = fe_add(p.z, txy), t = fe_sub(p.z, txy);
     mk_ext r;
     // t (limbs < 2^28) only ever on the first operand; x and z are each the second
     // operand of two products, so their 19x multiples are computed once
@@ -40,24 +46,24 @@ MK_DEV mk_ext mk_add_niels(const mk_ext& p, const mk_niels& q) {
     return r;
 }
 
-// The last addition as fractions: y = (pp + mm) / (2Z - T*2dxy) and
-// x = (pp - mm) / (2Z + T*2dxy), 3 multiplications (reference kernel.cu:1613-1627).
-// yd is left uncarried: it is mk_add_niels's t (limbs below 2^28 / 2^27 + 2^19.3), so
-// callers take it only as the first operand of fe_mul. xd is carried; yn and xn are
-// sums and differences of products (second-operand bounds).
+// The last addition as fractions: y = (pp + mm) / (Z - T*dxy) and
+// x = (pp - mm) / (Z + T*dxy), 3 multiplications (the reference's (pp + mm) / (2Z - T*2dxy),
+// kernel.cu:1613-1627, with numerator and denominator halved by the table).
+// yd is left uncarried: it is mk_add_niels's t (limbs below 2^27 + 2^26), so callers take
+// it only as the first operand of fe_mul. xd is carried; yn and xn are sums and
+// differences of products (second-operand bounds).
 MK_DEV void mk_add_niels_frac(const mk_ext& p, const mk_niels& q, fe& yn, fe& yd, fe& xn, fe& xd) {
     const fe pp = fe_mul(fe_add(p.y, p.x), q.ypx);
     const fe mm = fe_mul(fe_sub(p.y, p.x), q.ymx);
-    const fe txy = fe_mul(p.t, q.xy2d);
-    const fe z2 = fe_add(p.z, p.z);
+    const fe txy = fe_mul(p.t, q.xyd);
     yn = fe_add(pp, mm);
-    yd = fe_sub(z2, txy);
+    yd = fe_sub(p.z, txy);
     xn = fe_sub(pp, mm);
-    xd = fe_carry(fe_add(z2, txy));
+    xd = fe_carry(fe_add(p.z, txy));
 }
 
-// Row 0's entry (third field 2xy) to extended coordinates with no multiplication and no
-// carry pass: X = ypx + 2p - ymx = 2x, Y = ypx + ymx = 2y, Z = 2, T = 2xy. The point only
+// Row 0's entry ((y+x)/2, (y-x)/2, xy) to extended coordinates with no multiplication and
+// no carry pass: X = ypx + 2p - ymx = x, Y = ypx + ymx = y, Z = 1, T = xy. The point only
 // ever enters mk_add_niels as p, where Y + X = 2 ypx + 2p (limbs < 2^28) and
 // Y - X = Y + 2p - X = 2 ymx (exact limb by limb: no limb borrows, although X's limbs
 // exceed 2p's) are both first operands of fe_mul, as that addition requires.
@@ -65,8 +71,8 @@ MK_DEV mk_ext mk_row0_extended(const mk_niels& n) {
     mk_ext p;
     p.x = fe_sub(n.ypx, n.ymx);
     p.y = fe_add(n.ypx, n.ymx);
-    p.z = fe_small(2);
-    p.t = n.xy2d;
+    p.z = fe_small(1);
+    p.t = n.xyd;
     return p;
 }
 
@@ -156,13 +162,13 @@ MK_DEV mk_niels mk_entry_niels(const mk_entry& e, i32 d) {
     for (int i = 0; i < 10; i++) {
         n.ypx.v[i] = e.w[mk_entry_word(0, i)];
         n.ymx.v[i] = e.w[mk_entry_word(1, i)];
-        n.xy2d.v[i] = e.w[mk_entry_word(2, i)];
+        n.xyd.v[i] = e.w[mk_entry_word(2, i)];
     }
     if (d < 0) {
         const fe t = n.ypx;
         n.ypx = n.ymx;
         n.ymx = t;
-        n.xy2d = fe_neg(n.xy2d);
+        n.xyd = fe_neg(n.xyd);
     }
     return n;
 }
@@ -238,9 +244,9 @@ MK_DEV mk_niels mk_staged_niels(const uint4* stage, u32 b, u32 lane, i32 d) {
     for (int i = 0; i < 10; i++) {
         n.ypx.v[i] = ya[i];
         n.ymx.v[i] = yb[i];
-        n.xy2d.v[i] = yc[i];
+        n.xyd.v[i] = yc[i];
     }
-    if (neg) n.xy2d = fe_neg(n.xy2d);
+    if (neg) n.xyd = fe_neg(n.xyd);
     return n;
 }
 
@@ -293,6 +299,9 @@ MK_CONST u32 MK_BY[8] = {0x66666658u, 0x66666666u, 0x66666666u, 0x66666666u,
                          0x66666666u, 0x66666666u, 0x66666666u, 0x66666666u};
 MK_CONST u32 MK_D2[8] = {0x26b2f159u, 0xebd69b94u, 0x8283b156u, 0x00e0149au,
                          0xeef3d130u, 0x198e80f2u, 0x56dffce7u, 0x2406d9dcu};
+// 1/2 mod p = (p + 1) / 2 = 2^254 - 9.
+MK_CONST u32 MK_INV2[8] = {0xfffffff7u, 0xffffffffu, 0xffffffffu, 0xffffffffu,
+                           0xffffffffu, 0xffffffffu, 0xffffffffu, 0x3fffffffu};
 
 MK_DEV fe mk_const(const u32* w) {
     u32 t[8];
@@ -349,8 +358,8 @@ MK_DEV void mk_row_bases(u32 w, mk_ext* out) {
     }
 }
 
-// Entry (row, k) = k * base_row (+ 2^254 * B on the top row) as (y+x, y-x, 2dxy) in
-// canonical limbs, row 0 with 2xy as its third field; 32 words laid out by
+// Entry (row, k) = k * base_row (+ 2^254 * B on the top row) as ((y+x)/2, (y-x)/2, dxy)
+// in canonical limbs, row 0 with xy as its third field; 32 words laid out by
 // mk_entry_word (2 pad words zero).
 MK_DEV void mk_table_entry(u32 w, const mk_ext* bases, u32 row, u32 k, u32* out) {
     const u32 rows = mk_rows(w);
@@ -365,13 +374,14 @@ MK_DEV void mk_table_entry(u32 w, const mk_ext* bases, u32 row, u32 k, u32* out)
     const fe zi = fe_invert(acc.z);
     const fe x = fe_mul(acc.x, zi), y = fe_mul(acc.y, zi);
     const fe xy = fe_mul(x, y);
+    const fe half = mk_const(MK_INV2);
     fe parts[3];
-    parts[0] = fe_carry(fe_add(y, x));
-    parts[1] = fe_carry(fe_sub(y, x));
-    parts[2] = row == 0 ? fe_carry(fe_add(xy, xy)) : fe_mul(xy, d2);
+    parts[0] = fe_mul(fe_add(y, x), half);
+    parts[1] = fe_mul(fe_sub(y, x), half);
+    parts[2] = row == 0 ? xy : fe_mul(xy, fe_mul(d2, half));
     for (int f = 0; f < 3; f++) {
         u32 wds[8];
-        fe_to_words(parts[f], wds);
+        fe_carried_to_words(parts[f], wds);
         const fe c = fe_from_words(wds);  // canonical limbs
         for (int i = 0; i < 10; i++) out[mk_entry_word(f, i)] = c.v[i];
     }

@@ -96,10 +96,36 @@ def _entry_word(f: int, i: int) -> int:
     return 8 * f + i if i < 8 else 24 + 2 * f + (i - 8)
 
 
+_P_WORDS = [0xFFFFFFED] + [0xFFFFFFFF] * 6 + [0x7FFFFFFF]
+
+
+def _double_mod_p(v):
+    """2v mod p for canonical 256-bit values held as 8 little-endian 32-bit words in the
+    columns of a uint64 array (one value per row), vectorized over the rows."""
+    import numpy as np
+    m32 = np.uint64(0xFFFFFFFF)
+    d = np.zeros_like(v)
+    carry = np.zeros(v.shape[0], dtype=np.uint64)
+    for i in range(8):  # 2v < 2p < 2^256: no carry out of word 7
+        t = (v[:, i] << np.uint64(1)) | carry
+        d[:, i] = t & m32
+        carry = t >> np.uint64(32)
+    r = np.zeros_like(v)
+    borrow = np.zeros(v.shape[0], dtype=np.int64)
+    for i in range(8):  # 2v - p, word by word
+        t = d[:, i].astype(np.int64) - np.int64(_P_WORDS[i]) - borrow
+        borrow = (t < 0).astype(np.int64)
+        r[:, i] = (t + (borrow << 32)).astype(np.uint64)
+    keep = borrow.astype(bool)  # 2v < p
+    r[keep] = d[keep]
+    return r
+
+
 def ref_table_from_device(words):
-    """The reference's packed table (24 words per entry: three canonical 8-word LE
-    values, row 0's third field 2xy, gpu.rs:1455-1480) from the device table's entries
-    (numpy uint32, shape (entries, 32), canonical limbs)."""
+    """The reference's packed table (24 words per entry: (y+x, y-x, 2dxy) as canonical
+    8-word LE values, row 0's third field 2xy, gpu.rs:1455-1480) from the device table's
+    entries (numpy uint32, shape (entries, 32), canonical limbs of the halved fields
+    ((y+x)/2, (y-x)/2, dxy; row 0 xy): each field is doubled mod p)."""
     import numpy as np
     n = words.shape[0]
     out = np.zeros((n, 24), dtype=np.uint64)
@@ -110,6 +136,7 @@ def ref_table_from_device(words):
             out[:, 8 * f + w] |= limb & np.uint64(0xFFFFFFFF)
             if w + 1 < 8:
                 out[:, 8 * f + w + 1] |= limb >> np.uint64(32)
+        out[:, 8 * f:8 * f + 8] = _double_mod_p(out[:, 8 * f:8 * f + 8])
     flat = out.astype(np.uint32).reshape(-1)
     return (ctypes.c_uint32 * flat.size).from_buffer_copy(flat.tobytes())
 

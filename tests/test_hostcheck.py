@@ -145,13 +145,14 @@ def test_table_matches_oracle(oracle, hostcheck, w):
         words = list(tab[(row * cols + k) * 32:(row * cols + k + 1) * 32])
         ypx, ymx, third = _entry_ints(words)
         n = oracle.comb_entry_niels(w, row, k)
-        assert ypx == int.from_bytes(n[0:32], "little")
-        assert ymx == int.from_bytes(n[32:64], "little")
+        half = T.inv(2)  # every field halved: (y+x)/2, (y-x)/2, dxy
+        assert ypx == int.from_bytes(n[0:32], "little") * half % P
+        assert ymx == int.from_bytes(n[32:64], "little") * half % P
         xy2d = int.from_bytes(n[64:96], "little")
         if row == 0:
-            assert third == xy2d * T.inv(T.D) % P  # row 0 keeps 2xy
+            assert third == xy2d * T.inv(2 * T.D) % P  # row 0 holds xy
         else:
-            assert third == xy2d
+            assert third == xy2d * half % P
         assert words[30] == 0 and words[31] == 0
         # canonical limbs
         from tests.hostcheck import entry_limbs
