@@ -293,7 +293,7 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
     MK_HIP_G(hipFree(bases));
     bases = nullptr;
 
-    g->scratch_bytes = g->lanes * opt.batch * 30 * 4;
+    g->scratch_bytes = g->lanes * opt.batch * 20 * 4;  // two field elements per key (mk_lane_tile)
     MK_HIP_G(hipMalloc((void**)&g->scratch, g->scratch_bytes));
     MK_HIP_G(hipMalloc((void**)&g->ctr, sizeof(mk_counters)));
     MK_HIP_G(hipMalloc((void**)&g->cands, sizeof(mk_cand) * kCandCap));

@@ -122,7 +122,7 @@ int hc_lane(const u32* table, u32 w, const u32 key[8], u32 source, u64 first, u3
     mt.len = m.len;
     mt.mask = m.mask;
     mt.dfa = m.dfa.empty() ? nullptr : m.dfa.data();
-    std::vector<u32> spill((size_t)K * 30);
+    std::vector<u32> spill((size_t)K * 20);
     mk_spill sp;
     sp.base = spill.data();
     sp.stride = 1;

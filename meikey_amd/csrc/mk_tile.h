@@ -6,7 +6,8 @@
 #include "mk_hash.h"
 #include "mk_point.h"
 
-// Per-lane spill area for the K numerators, denominators and prefix products.
+// Per-lane spill area of the batched inversion: two field elements per key (the key's
+// numerator times the prefix product before it, and its denominator).
 // Element `i` of field slot `slot` lives at base[(slot * 10 + i) * stride]; with
 // stride = number of lanes and base offset by the lane, the words of one slot are
 // contiguous across a wave.
@@ -33,11 +34,18 @@ struct mk_spill {
 // Counter seeds (source 1) hash through the search's SHA-512 mid-state, which mid()
 // returns; the search kernel keeps it in LDS and reads it afresh for each key, so its
 // 22 registers are not held through the comb multiply.
+//
+// Montgomery's simultaneous inversion with the numerators folded in: with
+// P_j = yd_0 ... yd_j, the forward pass spills Q_j = yn_j * P_(j-1) and yd_j, and after one
+// inversion of P_(K-1) the back-sweep takes y_j = Q_j / P_j and steps 1/P_j down to
+// 1/P_(j-1) by a multiplication with yd_j. Four multiplications per key, as with the
+// textbook form that spills yn, yd and P_(j-1), but two spilled fields instead of three and
+// one multiplication between a key's loads and its y.
 template <class M, class P, class F, class G>
 MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const mk_spill& sp, M mul, P pre_sweep,
                          F on_key, G mid) {
     u32 upper[8];
-    fe pre = fe_small(1);  // prefix product of the yd (each yd is a first operand of fe_mul)
+    fe pre = fe_small(1);  // P_(j-1); each yd and yn is a first operand of fe_mul
     const u64 tail = (u64)key[7] << 32 | key[6];
     MK_NO_UNROLL
     for (u32 j = 0; j < K; j++) {
@@ -52,38 +60,25 @@ MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const m
         }
         fe yn, yd;
         mul(s, yn, yd);
+        sp.store(2 * j, fe_mul(yn, pre));
+        sp.store(2 * j + 1, yd);
         pre = fe_mul(yd, pre);
-        sp.store(3 * j, yn);
-        sp.store(3 * j + 1, yd);
-        sp.store(3 * j + 2, pre);
     }
-    fe inv = fe_invert(pre);
+    fe inv = fe_invert(pre);  // 1 / P_(K-1)
     pre_sweep();
-    // 1/yd_j = inv * pre_(j-1) with inv = 1/pre_j; then inv <- inv * yd_j = 1/pre_(j-1).
-    // Each key's three spilled values are loaded one key ahead, so their HBM latency
+    // Each key's two spilled values are loaded one key ahead, so their HBM latency
     // overlaps the previous key's multiplications and matching.
-    fe n_yn = sp.load(3 * (K - 1)), n_yd = inv, n_pre = inv;
-    if (K > 1) {
-        n_yd = sp.load(3 * (K - 1) + 1);
-        n_pre = sp.load(3 * (K - 2) + 2);
-    }
+    fe n_q = sp.load(2 * (K - 1)), n_yd = sp.load(2 * (K - 1) + 1);
     MK_NO_UNROLL
     for (u32 jj = K; jj-- > 0;) {
-        const fe yn = n_yn, yd = n_yd, pre_prev = n_pre;
+        const fe q = n_q, yd = n_yd;
         if (jj > 0) {
-            n_yn = sp.load(3 * (jj - 1));
-            if (jj > 1) {
-                n_yd = sp.load(3 * (jj - 1) + 1);
-                n_pre = sp.load(3 * (jj - 2) + 2);
-            }
-        }
-        fe zinv = inv;
-        if (jj > 0) {
-            zinv = fe_mul(inv, pre_prev);
-            inv = fe_mul(yd, inv);
+            n_q = sp.load(2 * (jj - 1));
+            n_yd = sp.load(2 * (jj - 1) + 1);
         }
         u32 pk[8];
-        fe_carried_to_words(fe_mul(yn, zinv), pk);
+        fe_carried_to_words(fe_mul(q, inv), pk);  // y_jj = Q_jj / P_jj
+        if (jj > 0) inv = fe_mul(yd, inv);          // 1 / P_(jj-1)
         on_key(jj, pk);
     }
 }
