@@ -181,6 +181,25 @@ def test_full_pk_and_lane(oracle, hostcheck, w):
         assert cand[0] == 1
 
 
+@pytest.mark.parametrize("K", [1, 2, 3, 64])
+def test_lane_batched_inversion_edges(oracle, hostcheck, K):
+    """The lane tile's batched inversion (Q_j = yn_j * P_(j-1) spilled with yd_j, one
+    inversion, the back-sweep y_j = Q_j / P_j) gives every key's y for a tile of one, two
+    and three keys, where the prefetch and the last step have no predecessor, and for 64."""
+    w = 5
+    tab = hostcheck.table(w)
+    rng = random.Random(300 + K)
+    key = rng.randbytes(32)
+    for src in (0, 1):
+        first = rng.getrandbits(40)
+        pks = oracle.derive(key, src, first, K)
+        needle = T.hay(pks[0:32])[10:13].encode()
+        ys, _ = hostcheck.lane(tab, w, key, src, first, K, needle, False, 0)
+        for j in range(K):
+            pk = pks[32 * j:32 * j + 32]
+            assert ys[j] == pk[:31] + bytes([pk[31] & 0x7F]), (K, src, j)
+
+
 def test_bitap_matcher_vs_oracle(oracle, hostcheck):
     rng = random.Random(24)
     alpha = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
