@@ -86,6 +86,34 @@ def test_table_entries_vs_oracle(gpu, oracle):
         assert third == (xy2d * T.inv(2 * T.D) % T.P if row == 0 else xy2d * half % T.P), (row, k)
 
 
+def test_device_keys_equal_the_reference_kernel(gpu):
+    """The reference's own kernel (kernel.cu compiled for the host into oracle/_ref)
+    derives the same public keys, x sign included, as the device: over the device's
+    W = 16 table repacked into the reference's layout, both seed sources, at index
+    ranges that start at 0, straddle 2^32 and sit past 2^40."""
+    import ctypes
+    import struct
+    import numpy as np
+    from oracle import oracle as O
+    R = O.ref()
+    if R is None:
+        pytest.skip("the reference kernel's host build (oracle/_ref) is absent")
+    info = gpu.info()
+    assert info.options.width == 16
+    rows = [gpu.table_words(r, 0, info.cols, as_array=True) for r in range(info.rows)]
+    table = O.ref_table_from_device(np.concatenate(rows))
+    batch = R.meikey_test_batch()
+    rng = random.Random(47)
+    for src in (0, 1):
+        key = rng.randbytes(32)
+        kw = (ctypes.c_uint32 * 8)(*struct.unpack("<8I", key))
+        for first in (0, 12345, (1 << 32) - batch // 2, (1 << 40) + 7):
+            say(f"reference derive src={src} first={first}")
+            out = ctypes.create_string_buffer(32 * batch)
+            R.meikey_test_derive_batch(table, 16, kw, src, first, out)
+            assert out.raw == gpu.derive(key, src, first, batch), (src, first)
+
+
 def test_search_golden(gpu):
     for case in GOLDEN["search"]:
         key = bytes.fromhex(case["key"])
