@@ -70,7 +70,7 @@ typedef struct {
     uint32_t grid_waves;    /* workgroups (one wave64 each) per launch */
     uint64_t hbm_bytes;
     uint64_t table_bytes;
-    uint64_t scratch_bytes; /* per-lane numerator/denominator buffer */
+    uint64_t scratch_bytes; /* per-lane buffer of the batched inversion (80 B per key) */
     mk_options options;
     uint32_t rows, cols;    /* comb geometry of the table */
 } mk_info;

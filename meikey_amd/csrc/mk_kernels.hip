@@ -9,11 +9,11 @@
 // an even index also gives the seed of the odd index after it. Per key:
 //   ChaCha20 -> SHA-512 + clamp -> comb multiply (rows - 2 extended+Niels additions and
 //   a 3-multiplication fraction for the last row) -> y numerator/denominator.
-// The K denominators of a lane are inverted together (Montgomery's trick: one
-// inversion, 3 multiplications per key). The numerators, denominators and prefix
-// products wait in a global buffer laid out [key][word][lane] so every access of a
-// wave is one contiguous 256-byte line (HBM traffic 240 B per key, against the
-// 16-20 KFLOP-equivalent of integer work per key). y is compressed with the x sign
+// The K denominators of a lane are inverted together (Montgomery's trick with the
+// numerators folded in, mk_lane_tile: one inversion, 4 multiplications per key). Two
+// field elements per key wait in a global buffer laid out [key][word][lane] so every
+// access of a wave is one contiguous 256-byte line (HBM traffic 160 B per key, against
+// about 16 k instructions of integer work per key). y is compressed with the x sign
 // still unknown and matched with sign_unknown (superset of the true hits); lanes
 // with a candidate append it through one wave ballot and one atomic.
 // The confirm kernel re-derives each candidate with x, matches exactly and writes
