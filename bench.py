@@ -119,7 +119,10 @@ def main():
                      # integer VALU is the bound (no dense contraction for MFMA); the HBM
                      # side of the same launches, from the committed traffic counters
                      "hbm": ({"achieved": traffic / (avg_ms * 1e-3) / 1e12, "peak": 8.0, "unit": "TB/s",
-                              "frac": traffic / (avg_ms * 1e-3) / 8e12} if traffic else None)},
+                              "frac": traffic / (avg_ms * 1e-3) / 8e12} if traffic else None),
+                     # the bound itself: how often the SIMDs issue a VALU instruction, from
+                     # the committed SQ counter run of the same launch
+                     "valu_issue": (profiled(args.width, args.batch, args.blocks) or {}).get("valu_issue")},
     }
     if not args.no_batch:
         out["batched"] = batched_leg(gpu, rate)
@@ -268,10 +271,10 @@ def serving_leg(gpu, seconds):
     return out, served
 
 
-def measured_traffic(width: int, batch: int, blocks: int):
-    """HBM bytes per launch of the search kernel, from the committed FETCH_SIZE /
-    WRITE_SIZE counter runs (profiles/mk_search_kernel_traffic.json, each scaled by the
-    same counter's reading for a copy kernel of known size); None when those runs were
+def profiled(width: int, batch: int, blocks: int):
+    """The committed counter runs of the search kernel (profiles/mk_search_kernel_traffic.json):
+    FETCH_SIZE / WRITE_SIZE, each scaled by the same counter's reading for a copy kernel
+    of known size, and the VALU issue rate of the SQ counters. None when those runs were
     made for another setting."""
     path = os.path.join(ROOT, "profiles", "mk_search_kernel_traffic.json")
     try:
@@ -280,7 +283,13 @@ def measured_traffic(width: int, batch: int, blocks: int):
         return None
     if (t.get("width"), t.get("keys_per_thread"), t.get("waves_per_simd")) != (width, batch, blocks):
         return None
-    return t["bytes_per_key"] * STEP_KEYS
+    return t
+
+
+def measured_traffic(width: int, batch: int, blocks: int):
+    """HBM bytes per launch of the search kernel, from the committed counter runs."""
+    t = profiled(width, batch, blocks)
+    return t["bytes_per_key"] * STEP_KEYS if t else None
 
 
 def copy_calibration(nbytes: int) -> None:
