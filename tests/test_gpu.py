@@ -398,6 +398,21 @@ def test_edge_inputs(gpu, oracle):
     assert [h.index for h in hits] == [target]
 
 
+@pytest.mark.parametrize("source", [0, 1])
+def test_index_range_across_2_64(gpu, oracle, source):
+    """A range that starts just below 2^64 continues at index 0 (u64 indices, as the
+    reference's index arithmetic wraps): the hit set equals the oracle's, hits on both
+    sides of the wrap, listed in ascending index order."""
+    key = random.Random(93 + source).randbytes(32)
+    first = (1 << 64) - 3000
+    hits, tries = gpu.batch_run([_spec(key, source, b"ab", False, M.MODE_CONTAINS, first, 6144)])
+    want = oracle.search(key, source, first, tries[0], b"ab", False, M.MODE_CONTAINS)
+    got = [h.index for h in hits]
+    assert got == sorted(got)
+    assert got == sorted(want)
+    assert any(i >= first for i in got) and any(i < 4096 for i in got)
+
+
 def test_one_key_per_lane_and_eight_waves(oracle):
     """The extremes of the launch shape: 1 key per lane (an inversion per key) and a grid
     sized for 8 waves per SIMD (more waves than fit, queued by the hardware)."""
