@@ -113,8 +113,11 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
     // lane, claimed one by one from a second counter, so the waves finish within a
     // fraction of a tile of each other. Units keep at least 64 keys per lane (the
     // inversion is shared by a unit's keys). A serving step of one tile per wave is
-    // therefore claimed entirely in units.
-    const u32 U = (K >= 128 && K % 64 == 0) ? K / 64 : 1;
+    // therefore claimed entirely in units; there they are 128 keys, which balance the
+    // step's mix of matchers at half the inversions of 64-key units (0.5% more served
+    // keys/s; whole tiles serve 5% fewer).
+    const u32 U = serve != nullptr ? ((K >= 128 && K % 128 == 0) ? K / 128 : 1)
+                                   : ((K >= 128 && K % 64 == 0) ? K / 64 : 1);
     const u64 tail = U > 1 ? (total_tiles < gridDim.x ? total_tiles : gridDim.x) : 0;
     const u64 big_tiles = total_tiles - tail;
     for (;;) {
