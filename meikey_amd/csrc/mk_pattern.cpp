@@ -33,7 +33,12 @@ namespace {
 
 const char kAlphabet[] = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
 constexpr u32 kMaxStates = 1u << 14;  // dfa.rs:20 MAX_STATES
-constexpr size_t kMaxNfa = 1u << 20;
+constexpr size_t kMaxNfa = 1u << 15;
+// Every pattern runs over the 43-character hay, so at most 43 copies of a repeated part
+// consume a character; any others match empty at a position where any number of copies
+// can. x{n,m} therefore matches the same hays as x{n,m} with both counts cut to 44, and
+// with m >= 44 as x{min(n, 44),}: a count of 100000 builds 44 copies, not 100000.
+constexpr int kRepeatCap = 44;
 constexpr int kHayLen = 43;           // b64[25..68]
 constexpr int kFreeLen = 42;          // Pattern::MAX_LEN (pattern.rs:21, 112)
 
@@ -602,7 +607,7 @@ class NfaBuilder {
     const std::vector<Node>& nodes_;
 
     int add(NState s) {
-        if (st.size() >= kMaxNfa) throw regex_error{"too complex"};
+        if (st.size() >= kMaxNfa) throw regex_error{"too complex nfa"};
         st.push_back(s);
         return (int)st.size() - 1;
     }
@@ -655,14 +660,19 @@ class NfaBuilder {
             default: {
                 // N_REPEAT: min copies, then (max - min) optional copies or a star
                 const int kid = n.kids[0];
+                int rmin = n.min, rmax = n.max;
+                if (rmax < 0 || rmax >= kRepeatCap) {
+                    rmax = -1;
+                    rmin = std::min(rmin, kRepeatCap);
+                }
                 int e = add({S_EPS});
                 Frag f{e, {&st[e].out}};
-                for (int i = 0; i < n.min; i++) {
+                for (int i = 0; i < rmin; i++) {
                     Frag g = frag(kid);
                     patch(f, g.start);
                     f.dangling = g.dangling;
                 }
-                if (n.max < 0) {
+                if (rmax < 0) {
                     Frag g = frag(kid);
                     int s = add({S_SPLIT});
                     st[s].out = g.start;
@@ -671,7 +681,7 @@ class NfaBuilder {
                     return {e, {&st[s].out2}};
                 }
                 std::vector<int*> outs;
-                for (int i = n.min; i < n.max; i++) {
+                for (int i = rmin; i < rmax; i++) {
                     Frag g = frag(kid);
                     int s = add({S_SPLIT});
                     st[s].out = g.start;
@@ -867,6 +877,9 @@ mk_status mk_pattern_new(const char* text, uint32_t text_len, int case_sensitive
             if (e.msg == "too complex")
                 return perr(error_kind, MK_PATTERN_TOO_COMPLEX,
                             "the regular expression is too complex (the DFA has more than %u states)", kMaxStates);
+            if (e.msg == "too complex nfa")
+                return perr(error_kind, MK_PATTERN_TOO_COMPLEX,
+                            "the regular expression is too complex (its automaton has more than %zu states)", kMaxNfa);
             return perr(error_kind, MK_PATTERN_REGEX, "invalid regular expression: %s", e.msg.c_str());
         }
         p->probability = dfa_probability(p->dfa);

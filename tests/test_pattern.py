@@ -233,6 +233,50 @@ def test_dfa_equals_python_re():
     assert tested > 250 and hits > 2000
 
 
+@pytest.mark.parametrize("text,same", [
+    ("(a?){100}b", "a{0,100}b"), ("x(y?){45,}z", "xy*z"), ("(ab|c){2,60}", "(?:ab|c){2,60}"),
+    ("[A-P]{0,1000}x", "[A-P]*x"), ("(a|b|c){3,500}$", "[abc]{3,}$"), ("^(.){43,50}", "^.{43}"),
+    ("(.){43}", ".{43}"), ("((a?b?){50}){50}c", "[ab]*c")])
+def test_large_repetition_counts_match_python_re(text, same):
+    """Counted repetitions above 44 are built as 44 copies (the hay has 43 characters).
+    The DFA agrees with re.search of an equivalent pattern that Python's backtracking
+    matcher runs in linear time, on every hay; and it compiles in well under a second."""
+    import time
+    rng = random.Random(zlib.crc32(text.encode()))
+    t0 = time.perf_counter()
+    p = M.Pattern(text, True, M.MODE_REGEX)
+    assert time.perf_counter() - t0 < 2.0
+    rx = re.compile(same, re.ASCII)
+    table, n, _, _ = p.needle()
+    small = sorted(set(c for c in text if c in ALPHA) | set("aA0+B"))
+    hits = 0
+    for i in range(200):
+        if i % 2:
+            hay = random_hay(rng)
+        else:
+            hay = rng.choice("ABCDEFGHIJKLMNOP") + "".join(rng.choice(small) for _ in range(42))
+        want = rx.search(hay) is not None
+        hits += want
+        assert dfa_walk(table, [ALPHA.index(c) for c in hay]) == want, (text, hay)
+    assert hits > 0
+
+
+@pytest.mark.parametrize("text,kind", [("(.){44}", "NeverMatches"), ("(.){45,1000}", "NeverMatches"),
+                                       ("(\\b.){60}", "NeverMatches"),
+                                       ("(a{1000}){1000}", "NeverMatches"), ("(a|b|c){100000}", "NeverMatches"),
+                                       ("((((a{44}){44}){44}){44})b", "TooComplex")])
+def test_huge_repetitions_fail_fast(text, kind):
+    """Patterns that need more than the 43 hay characters never match, whatever their
+    counts, and nested counts that would build a huge automaton are too complex: each is
+    rejected within seconds, not after building millions of states."""
+    import time
+    t0 = time.perf_counter()
+    with pytest.raises(M.PatternError) as e:
+        M.Pattern(text, True, M.MODE_REGEX)
+    assert e.value.kind == kind, e.value
+    assert time.perf_counter() - t0 < 5.0
+
+
 @pytest.mark.parametrize("text", ["ab+c", "[0-9][A-F]x", "^[A-P]a|z$", "(ab){2,}", "\\b9", "a.b"])
 @pytest.mark.parametrize("cs", [False, True])
 def test_matches_regex_crate_cases(text, cs):
