@@ -161,7 +161,12 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
             tile_first = sg.first + (tile - sg.tile_begin) * 64ull * K + sub * 64ull * Keff;
         }
         const mk_search_desc& dsc = descs[search];
-        if (dsc.stop_on_hit && __atomic_load_n(&stop[search], __ATOMIC_RELAXED) != 0) continue;
+        // A serving unit has already advanced its request's next index, so it is searched
+        // even if the request stopped meanwhile: skipping it would leave a hole of
+        // unsearched indices that still count as tries (the stop flag was checked before
+        // the claim, and the request may go on after a candidate that the exact check
+        // rejects).
+        if (serve == nullptr && dsc.stop_on_hit && __atomic_load_n(&stop[search], __ATOMIC_RELAXED) != 0) continue;
         if (search != cached) {
             __syncthreads();
             lmask[lane] = masks[(u64)dsc.mask_off + lane];

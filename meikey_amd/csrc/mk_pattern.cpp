@@ -546,7 +546,10 @@ class Parser {
     }
 };
 
-// Shortest match length (Hir::minimum_len): look-arounds count zero.
+// Shortest match length (Hir::minimum_len): look-arounds count zero. Saturates at
+// kMinLenCap (every length above the 43-character hay is alike here), so nested counts
+// never overflow: regex-syntax computes the same with checked arithmetic.
+constexpr long kMinLenCap = kHayLen + 1;
 long min_len(const std::vector<Node>& nodes, int id) {
     const Node& n = nodes[id];
     switch (n.kind) {
@@ -554,7 +557,7 @@ long min_len(const std::vector<Node>& nodes, int id) {
         case N_EMPTY: case N_ASSERT: return 0;
         case N_CONCAT: {
             long t = 0;
-            for (int k : n.kids) t += min_len(nodes, k);
+            for (int k : n.kids) t = std::min(kMinLenCap, t + min_len(nodes, k));
             return t;
         }
         case N_ALT: {
@@ -565,7 +568,7 @@ long min_len(const std::vector<Node>& nodes, int id) {
             }
             return m < 0 ? 0 : m;
         }
-        default: return (long)n.min * min_len(nodes, n.kids[0]);
+        default: return std::min(kMinLenCap, (long)n.min * min_len(nodes, n.kids[0]));
     }
 }
 
@@ -719,10 +722,23 @@ struct Dfa {
 // Subset construction. DFA state = (NFA set-states waiting to consume a character,
 // context of the character before). `words` says whether \b or \B occur; if not, the
 // word / non-word context is merged.
+//
+// The closure of a state depends on the value read only through that value's context
+// (word or non-word, one class when `words` is false), so it is computed once per
+// context and the 64 transitions are read off it. The work is bounded for untrusted
+// pattern text like regex-automata's determinize_size_limit (the reference's 16 MiB
+// BUILD_LIMIT, src/model/pattern/dfa.rs:23, 64-65): the bytes of the state keys kept
+// and the closure steps taken are counted, and past the limit the pattern is too complex.
+constexpr size_t kBuildLimit = 16u << 20;        // bytes of DFA state keys
+constexpr u64 kClosureSteps = 1ull << 25;        // NFA states visited, summed over closures
+constexpr size_t kStateKeyOverhead = 64;         // map node and vector header per state
+
 Dfa determinize(const std::vector<NState>& nfa, int start, bool words) {
     std::vector<int> stack;
     std::vector<u32> mark(nfa.size(), 0);
     u32 stamp = 0;
+    u64 steps = 0;
+    size_t key_bytes = 0;
     // closure of `seeds` plus the start state in context (prev, next): SET states and
     // whether MATCH is reached
     auto closure = [&](const std::vector<int>& seeds, int prev, int next, std::vector<int>& sets) {
@@ -736,6 +752,7 @@ Dfa determinize(const std::vector<NState>& nfa, int start, bool words) {
             stack.pop_back();
             if (s < 0 || mark[s] == stamp) continue;
             mark[s] = stamp;
+            if (++steps > kClosureSteps) throw regex_error{"too complex build"};
             const NState& n = nfa[s];
             switch (n.kind) {
                 case S_SET: sets.push_back(s); break;
@@ -750,10 +767,12 @@ Dfa determinize(const std::vector<NState>& nfa, int start, bool words) {
                     break;
             }
         }
-        std::sort(sets.begin(), sets.end());
         return match;
     };
     auto ctx_of = [&](int v) { return words ? (is_word_value(v) ? 1 : 2) : 1; };
+    // the values of each next-character context (1 word, 2 non-word)
+    u64 ctx_values[3] = {0, 0, 0};
+    for (int v = 0; v < 64; v++) ctx_values[ctx_of(v)] |= 1ull << v;
 
     Dfa d;
     std::map<std::vector<int>, u32> ids;  // key: sorted waiting states + [prev context]
@@ -765,31 +784,49 @@ Dfa determinize(const std::vector<NState>& nfa, int start, bool words) {
     ids[first] = 1;
     order.push_back(first);
     std::vector<int> sets, key;
+    std::vector<int> outs[64];
     for (size_t i = 0; i < order.size(); i++) {
         const std::vector<int> cur = order[i];
         const int prev = cur.back();
         const std::vector<int> waiting(cur.begin(), cur.end() - 1);
-        for (int v = 0; v < 64; v++) {
-            u16 id;
-            if (closure(waiting, prev, ctx_of(v), sets)) {
-                id = 0;
-            } else {
-                key.clear();
-                for (int s : sets)
-                    if ((nfa[s].set >> v) & 1) key.push_back(nfa[s].out);
+        u16 row[64];
+        for (int ctx = 1; ctx <= 2; ctx++) {
+            const u64 vals = ctx_values[ctx];
+            if (vals == 0) continue;
+            if (closure(waiting, prev, ctx, sets)) {
+                for (int v = 0; v < 64; v++)
+                    if ((vals >> v) & 1) row[v] = 0;
+                continue;
+            }
+            for (int v = 0; v < 64; v++) outs[v].clear();
+            for (int s : sets) {
+                u64 m = nfa[s].set & vals;
+                steps += (u64)__builtin_popcountll(m);
+                while (m) {
+                    const int v = __builtin_ctzll(m);
+                    m &= m - 1;
+                    outs[v].push_back(nfa[s].out);
+                }
+            }
+            if (steps > kClosureSteps) throw regex_error{"too complex build"};
+            for (int v = 0; v < 64; v++) {
+                if (!((vals >> v) & 1)) continue;
+                key = outs[v];
                 std::sort(key.begin(), key.end());
                 key.erase(std::unique(key.begin(), key.end()), key.end());
-                key.push_back(ctx_of(v));
+                key.push_back(ctx);
                 auto it = ids.find(key);
                 if (it == ids.end()) {
                     if (order.size() + 1 >= kMaxStates) throw regex_error{"too complex"};
+                    key_bytes += key.size() * sizeof(int) + kStateKeyOverhead;
+                    if (key_bytes > kBuildLimit) throw regex_error{"too complex build"};
                     it = ids.emplace(key, (u32)order.size() + 1).first;
                     order.push_back(key);
                 }
-                id = (u16)it->second;
+                row[v] = (u16)it->second;
             }
-            d.trans.push_back(id);
         }
+        d.trans.insert(d.trans.end(), row, row + 64);
         d.eoi.push_back(closure(waiting, prev, 0, sets) ? 1 : 0);
     }
     return d;
@@ -880,6 +917,10 @@ mk_status mk_pattern_new(const char* text, uint32_t text_len, int case_sensitive
             if (e.msg == "too complex nfa")
                 return perr(error_kind, MK_PATTERN_TOO_COMPLEX,
                             "the regular expression is too complex (its automaton has more than %zu states)", kMaxNfa);
+            if (e.msg == "too complex build")
+                return perr(error_kind, MK_PATTERN_TOO_COMPLEX,
+                            "the regular expression is too complex (building its DFA exceeds the %zu MiB limit)",
+                            kBuildLimit >> 20);
             return perr(error_kind, MK_PATTERN_REGEX, "invalid regular expression: %s", e.msg.c_str());
         }
         p->probability = dfa_probability(p->dfa);

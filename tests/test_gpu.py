@@ -433,10 +433,42 @@ def config4_specs(count):
     return W.batched(count)
 
 
+def _planted_twin(pat, key, index, oracle, rng):
+    """A pattern of the same kind, case rule and length as `pat` that the key at `index`
+    matches: a slice of that key's text (for a regex, the slice as an escaped literal,
+    anchored where `pat` is)."""
+    hay = T.hay(oracle.derive(key, 0, index, 1))
+    if pat.mode == M.MODE_REGEX:
+        n = 4
+        if pat.text.startswith("^"):
+            body, pre, post = hay[:n], "^", ""
+        elif pat.text.endswith("$"):
+            body, pre, post = hay[-n:], "", "$"
+        else:
+            o = rng.randrange(43 - n + 1)
+            body, pre, post = hay[o:o + n], "", ""
+        return M.Pattern(pre + body.replace("+", "\\+") + post, pat.case_sensitive, M.MODE_REGEX)
+    n = len(pat.text)
+    if pat.mode == M.MODE_SUFFIX:
+        return M.Pattern(hay[-n:], pat.case_sensitive, M.MODE_SUFFIX)
+    o = rng.randrange(43 - n + 1)
+    return M.Pattern(hay[o:o + n], pat.case_sensitive, M.MODE_CONTAINS)
+
+
 def test_config4_batched_equals_single_and_oracle(gpu, oracle):
-    """64 concurrent searches of mixed kinds in one launch: per-search hit sets equal
-    to one search per launch over the full 2^22 range, and to the oracle over the
-    first 2^15 candidates of every search."""
+    """64 concurrent searches of mixed kinds in one launch (config 4), 2^22 candidates
+    each, then checked three ways:
+    - per-search hit sets equal one search per launch over the full range;
+    - every hit of every search is re-derived (seed and public key) and re-matched by the
+      oracle;
+    - around up to two hits of each search, a 2^14-candidate window has exactly the
+      oracle's hit set (nothing missed there).
+    Long patterns have no hit in 2^22 candidates (a 7-char case-sensitive substring needs
+    about 10^11), so each search without one gets a planted twin in the same launch: the
+    same key, kind, case rule and length, with the text of a chosen key as its pattern;
+    its window around the planted key is checked the same way. Every one of the 64
+    search configurations is thereby checked on at least one oracle-confirmed hit
+    (src/service/gpu.rs:2155-2229, match agreement)."""
     n = 1 << 22
     specs, pats, keys = config4_specs(n)
     say("config4 batched")
@@ -444,14 +476,47 @@ def test_config4_batched_equals_single_and_oracle(gpu, oracle):
     assert all(t == n for t in tries)
     by_search = {i: [] for i in range(64)}
     for h in batched:
-        by_search[h.search].append(h.index)
+        by_search[h.search].append(h)
     for i, s in enumerate(specs):
         single, _ = gpu.batch_run([s])
-        assert by_search[i] == [h.index for h in single], i
-        sub = 1 << 15
-        want = oracle.search(keys[i], 0, 0, sub, s["needle"], not s["fold"], s["mode"])
-        assert [x for x in by_search[i] if x < sub] == want, i
-    say(f"config4 hits={len(batched)}")
+        assert [h.index for h in by_search[i]] == [h.index for h in single], i
+    for i, s in enumerate(specs):
+        cs = not s["fold"]
+        for h in by_search[i]:
+            assert h.seed == oracle.seed(keys[i], 0, h.index), (i, h.index)
+            assert h.pk == oracle.derive(keys[i], 0, h.index, 1), (i, h.index)
+            assert oracle.match(oracle.public_b64(h.pk), s["needle"], cs, s["mode"]), (i, h.index)
+    rng = random.Random(48)
+    win = 1 << 14
+
+    def window_check(i, spec, hits, around):
+        lo = max(0, min(around - win // 2, n - win))
+        want = oracle.search(spec["key"], 0, lo, win, spec["needle"], not spec["fold"], spec["mode"])
+        assert around in want, (i, around)
+        assert [h.index for h in hits if lo <= h.index < lo + win] == want, (i, lo)
+
+    twins = {}
+    for i, s in enumerate(specs):
+        hs = by_search[i]
+        if hs:
+            for h in rng.sample(hs, min(2, len(hs))):
+                window_check(i, s, hs, h.index)
+        else:
+            planted = rng.randrange(n)
+            twins[i] = (planted, _planted_twin(pats[i], keys[i], planted, oracle, rng).spec(keys[i], 0, n))
+    say(f"config4 hits={len(batched)}; planted twins for {len(twins)} searches")
+    if twins:
+        order = sorted(twins)
+        thits, ttries = gpu.batch_run(specs + [twins[i][1] for i in order])
+        assert all(t == n for t in ttries)
+        for j, i in enumerate(order):
+            planted, spec = twins[i]
+            hs = [h for h in thits if h.search == 64 + j]
+            for h in hs:
+                assert h.pk == oracle.derive(keys[i], 0, h.index, 1), (i, h.index)
+            window_check(i, spec, hs, planted)
+        # the original 64 searches are unchanged by the extra ones in the launch
+        assert [(h.search, h.index) for h in thits if h.search < 64] == [(h.search, h.index) for h in batched]
 
 
 def _served_checks(oracle, spec, r):
