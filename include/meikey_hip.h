@@ -129,6 +129,17 @@ mk_status mk_free_launch(mk_launch* l);
 mk_status mk_derive(mk_gpu* g, const uint8_t key[32], int seed_source, uint64_t first, uint64_t n, uint8_t* pks,
                     uint8_t* seeds);
 
+/* Allocation-free mk_derive. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The keys are staged in a fixed device array of the library (2 x 2 MiB
+ * per device, held until the process ends), so the call makes no hipMalloc / hipFree; it
+ * takes at most MK_DERIVE_NOALLOC_MAX keys (MK_ERR_OUT_OF_RANGE beyond). All calls on one
+ * GPU share that GPU's array: calls that overlap on one GPU (several threads or handles)
+ * race, and the caller must serialize them. pks / seeds (optional) as in mk_derive. */
+#define MK_DERIVE_NOALLOC_MAX 65536u
+mk_status mk_derive_noalloc(mk_gpu* g, const uint8_t key[32], int seed_source, uint64_t first, uint32_t n,
+                            uint8_t* pks, uint8_t* seeds);
+
 /* Public keys of explicit seeds (n x 32 bytes in, n x 32 bytes out). */
 mk_status mk_derive_seeds(mk_gpu* g, const uint8_t* seeds, uint32_t n, uint8_t* pks);
 

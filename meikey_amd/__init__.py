@@ -36,7 +36,8 @@ STATUS = {
 # Every entry point declared in include/meikey_hip.h.
 EXPORTS = (
     "mk_last_error", "mk_device_count", "mk_default_options", "mk_open", "mk_info_get", "mk_close",
-    "mk_keys_per_iter", "mk_prepare", "mk_launch_run", "mk_free_launch", "mk_derive", "mk_derive_seeds",
+    "mk_keys_per_iter", "mk_prepare", "mk_launch_run", "mk_free_launch", "mk_derive", "mk_derive_noalloc",
+    "mk_derive_seeds",
     "mk_table_read", "mk_batch_run", "mk_tile_keys", "mk_last_timing",
     "mk_pattern_new", "mk_pattern_last_error", "mk_pattern_free", "mk_pattern_needle", "mk_pattern_needle_bytes",
     "mk_pattern_probability", "mk_pattern_dfa_states", "mk_pattern_is_match",
@@ -151,6 +152,7 @@ def lib() -> ctypes.CDLL:
     L.mk_launch_run.argtypes = [P, U64, U32, ctypes.POINTER(HitRec), U32, ctypes.POINTER(U32)]
     L.mk_free_launch.argtypes = [P]
     L.mk_derive.argtypes = [P, ctypes.c_char_p, I32, U64, U64, P, P]
+    L.mk_derive_noalloc.argtypes = [P, ctypes.c_char_p, I32, U64, U32, P, P]
     L.mk_derive_seeds.argtypes = [P, ctypes.c_char_p, U32, P]
     L.mk_table_read.argtypes = [P, U32, U32, U32, P]
     L.mk_last_timing.argtypes = [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
@@ -421,6 +423,15 @@ class Gpu:
         pks = ctypes.create_string_buffer(max(32 * n, 1))
         sd = ctypes.create_string_buffer(max(32 * n, 1)) if seeds else None
         _check(lib().mk_derive(self._h, key, seed_source, first, n, pks, sd))
+        if seeds:
+            return pks.raw[: 32 * n], sd.raw[: 32 * n]
+        return pks.raw[: 32 * n]
+
+    def derive_noalloc(self, key: bytes, seed_source: int, first: int, n: int, seeds: bool = False):
+        """mk_derive_noalloc: derive without any device allocation (n <= 65536)."""
+        pks = ctypes.create_string_buffer(max(32 * n, 1))
+        sd = ctypes.create_string_buffer(max(32 * n, 1)) if seeds else None
+        _check(lib().mk_derive_noalloc(self._h, key, seed_source, first, n, pks, sd))
         if seeds:
             return pks.raw[: 32 * n], sd.raw[: 32 * n]
         return pks.raw[: 32 * n]

@@ -84,7 +84,7 @@ mk_status mk_run_step(mk_gpu* g, const mk_search_desc* descs, const u64* masks, 
     MK_HIP(hipEventRecord(g->ev[0], g->stream));
     if (total_tiles > 0)
         MK_HIP(mk_k_search(g->stream, g->grid, g->table, g->opt.width, g->opt.batch, descs, segs, nsegs, total_tiles,
-                           masks, dfas, g->scratch, g->ctr, g->cands, kCandCap, stop, nullptr, nullptr));
+                           masks, dfas, g->spill, g->ctr, g->cands, kCandCap, stop, nullptr, nullptr));
     MK_HIP(hipEventRecord(g->ev[1], g->stream));
     if (total_tiles > 0)
         MK_HIP(mk_k_confirm(g->stream, g->table, g->opt.width, descs, masks, dfas, g->ctr, g->cands, kCandCap, g->hits,
@@ -278,6 +278,7 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
     g->rows = rows_of(opt.width);
     g->cols = cols_of(opt.width);
     MK_HIP_G(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    MK_HIP_G(mk_k_noalloc_scratch(ordinal, &g->noalloc));
     for (auto& e : g->ev) MK_HIP_G(hipEventCreate(&e));
 
     // comb table, built on the device
@@ -293,8 +294,8 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
     MK_HIP_G(hipFree(bases));
     bases = nullptr;
 
-    g->scratch_bytes = g->lanes * opt.batch * 20 * 4;  // two field elements per key (mk_lane_tile)
-    MK_HIP_G(hipMalloc((void**)&g->scratch, g->scratch_bytes));
+    g->spill_bytes = g->lanes * opt.batch * 20 * 4;  // two field elements per key (mk_lane_tile)
+    MK_HIP_G(hipMalloc((void**)&g->spill, g->spill_bytes));
     MK_HIP_G(hipMalloc((void**)&g->ctr, sizeof(mk_counters)));
     MK_HIP_G(hipMalloc((void**)&g->cands, sizeof(mk_cand) * kCandCap));
     MK_HIP_G(hipMalloc((void**)&g->hits, sizeof(mk_hit) * kHitCap));
@@ -315,7 +316,7 @@ mk_status mk_info_get(const mk_gpu* g, mk_info* out) {
     out->grid_waves = g->grid;
     out->hbm_bytes = g->hbm;
     out->table_bytes = g->table_bytes;
-    out->scratch_bytes = g->scratch_bytes;
+    out->scratch_bytes = g->spill_bytes;
     out->options = g->opt;
     out->rows = g->rows;
     out->cols = g->cols;
@@ -327,7 +328,7 @@ mk_status mk_close(mk_gpu* g) {
     (void)hipSetDevice(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->table) (void)hipFree(g->table);
-    if (g->scratch) (void)hipFree(g->scratch);
+    if (g->spill) (void)hipFree(g->spill);
     if (g->ctr) (void)hipFree(g->ctr);
     if (g->cands) (void)hipFree(g->cands);
     if (g->hits) (void)hipFree(g->hits);
@@ -461,6 +462,33 @@ mk_status mk_derive(mk_gpu* g, const uint8_t key[32], int seed_source, uint64_t 
     if (dpk) (void)hipFree(dpk);
     if (dseed) (void)hipFree(dseed);
     if (e != hipSuccess) return fail(MK_ERR_DRIVER, "derive: %s", hipGetErrorString(e));
+    return MK_OK;
+}
+
+static_assert(MK_DERIVE_NOALLOC_MAX == MK_NOALLOC_KEYS, "the header's limit is the staging array's size");
+
+mk_status mk_derive_noalloc(mk_gpu* g, const uint8_t key[32], int seed_source, uint64_t first, uint32_t n,
+                            uint8_t* pks, uint8_t* seeds) {
+    if (!g || !key || (n && !pks)) return fail(MK_ERR_INVALID, "null pointer");
+    if (seed_source != MK_SEED_CHACHA20 && seed_source != MK_SEED_COUNTER)
+        return fail(MK_ERR_INVALID, "unknown seed source %d", seed_source);
+    if (n > MK_DERIVE_NOALLOC_MAX)
+        return fail(MK_ERR_OUT_OF_RANGE,
+                    "mk_derive_noalloc derives at most %u keys per call (%u asked): its device staging is a fixed "
+                    "array", MK_DERIVE_NOALLOC_MAX, n);
+    if (n == 0) return MK_OK;
+    if (!g->noalloc) return fail(MK_ERR_DRIVER, "the staging array of mk_derive_noalloc has no device address");
+    MK_HIP(hipSetDevice(g->device));
+    mk_key_words kw;
+    mk_key_to_words(key, kw.w);
+    // public keys in the first half of the array, seeds in the second; every word read
+    // back is written by this call's kernel, so nothing is cleared
+    u32* dpk = g->noalloc;
+    u32* dseed = seeds ? g->noalloc + 8 * MK_NOALLOC_KEYS : nullptr;
+    MK_HIP(mk_k_derive_kv(g->stream, g->table, g->opt.width, kw, (u32)seed_source, first, n, dpk, dseed));
+    MK_HIP(hipMemcpyAsync(pks, dpk, (size_t)n * 32, hipMemcpyDeviceToHost, g->stream));
+    if (seeds) MK_HIP(hipMemcpyAsync(seeds, dseed, (size_t)n * 32, hipMemcpyDeviceToHost, g->stream));
+    MK_HIP(hipStreamSynchronize(g->stream));
     return MK_OK;
 }
 

@@ -52,8 +52,9 @@ struct mk_gpu {
     char name[64] = {};
     u32* table = nullptr;
     u64 table_bytes = 0;
-    u32* scratch = nullptr;
-    u64 scratch_bytes = 0;
+    u32* spill = nullptr;  // per-lane buffer of the batched inversion
+    u32* noalloc = nullptr;  // this device's staging array of mk_derive_noalloc (not owned)
+    u64 spill_bytes = 0;
     mk_counters* ctr = nullptr;
     mk_cand* cands = nullptr;
     mk_hit* hits = nullptr;

@@ -62,3 +62,15 @@ struct mk_counters {
     u32 overflow;   // set when candidates or hits did not fit
     u32 pad;
 };
+
+// A search key's eight little-endian words, passed to a kernel by value.
+struct mk_key_words {
+    u32 w[8];
+};
+
+// Staging of the allocation-free derivation: public keys and seeds of at most
+// MK_NOALLOC_KEYS indices per call (2 x 2 MiB).
+#define MK_NOALLOC_KEYS (1u << 16)
+#define MK_NOALLOC_WORDS (2u * 8u * MK_NOALLOC_KEYS)
+// Devices whose staging address is kept.
+#define MK_MAX_DEVICES 64

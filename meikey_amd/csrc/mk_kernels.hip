@@ -48,6 +48,18 @@ extern "C" __global__ void __launch_bounds__(64) mk_table_fill_kernel(u32 w, con
 
 // --------------------------------------------------------------- derivation
 
+// Public key (and optionally seed) number i of indices first, first + 1, ...
+MK_DEV void mk_derive_one(const u32* __restrict__ table, u32 w, const u32 k[8], u32 source, u64 first, u64 i,
+                          u32* __restrict__ out_pk, u32* __restrict__ out_seed) {
+    u32 upper[8], seed[8], s[8], pk[8];
+    mk_seed_at(k, source, first + i, upper, true, seed);
+    mk_secret_scalar(seed, s);
+    mk_full_pk(table, w, s, pk);
+    for (int j = 0; j < 8; j++) out_pk[i * 8 + j] = pk[j];
+    if (out_seed)
+        for (int j = 0; j < 8; j++) out_seed[i * 8 + j] = seed[j];
+}
+
 // Public keys (and optionally seeds) of indices first .. first + n - 1 of one search
 // key. One thread per key.
 extern "C" __global__ void __launch_bounds__(64) mk_derive_kernel(const u32* __restrict__ table, u32 w,
@@ -56,15 +68,25 @@ extern "C" __global__ void __launch_bounds__(64) mk_derive_kernel(const u32* __r
                                                                   u32* __restrict__ out_seed) {
     const u64 i = (u64)blockIdx.x * 64 + threadIdx.x;
     if (i >= n) return;
-    u32 k[8], upper[8], seed[8], s[8], pk[8];
+    u32 k[8];
     for (int j = 0; j < 8; j++) k[j] = key[j];
-    mk_seed_at(k, source, first + i, upper, true, seed);
-    mk_secret_scalar(seed, s);
-    mk_full_pk(table, w, s, pk);
-    for (int j = 0; j < 8; j++) out_pk[i * 8 + j] = pk[j];
-    if (out_seed)
-        for (int j = 0; j < 8; j++) out_seed[i * 8 + j] = seed[j];
+    mk_derive_one(table, w, k, source, first, i, out_pk, out_seed);
 }
+
+// The same with the key passed by value, so the launch needs no device copy of it.
+extern "C" __global__ void __launch_bounds__(64) mk_derive_kv_kernel(const u32* __restrict__ table, u32 w,
+                                                                     mk_key_words key, u32 source, u64 first,
+                                                                     u64 n, u32* __restrict__ out_pk,
+                                                                     u32* __restrict__ out_seed) {
+    const u64 i = (u64)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    mk_derive_one(table, w, key.w, source, first, i, out_pk, out_seed);
+}
+
+// Staging of the allocation-free derivation (mk_derive_noalloc): MK_NOALLOC_KEYS public
+// keys, then as many seeds. File scope, one copy per device for the life of the process;
+// host code reaches it only through the address hipGetSymbolAddress gives per device.
+static __device__ __align__(256) u32 scratch[MK_NOALLOC_WORDS];
 
 // Public keys of explicit 32-byte seeds (known-answer self-check).
 extern "C" __global__ void __launch_bounds__(64) mk_derive_seeds_kernel(const u32* __restrict__ table, u32 w,
@@ -99,7 +121,7 @@ __device__ static u64 mk_uniform64(u64 v) {
 extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
     mk_search_kernel(const u32* __restrict__ table, u32 w, u32 K, const mk_search_desc* __restrict__ descs,
                      const mk_seg* __restrict__ segs, u32 nsegs, u64 total_tiles, const u64* __restrict__ masks,
-                     const u16* __restrict__ dfas, u32* __restrict__ scratch, mk_counters* __restrict__ ctr,
+                     const u16* __restrict__ dfas, u32* __restrict__ spill, mk_counters* __restrict__ ctr,
                      mk_cand* __restrict__ cands, u32 cand_cap, u32* __restrict__ stop,
                      const mk_serve_state* __restrict__ serve, u64* __restrict__ next_index) {
     __shared__ u64 lmask[64];
@@ -192,7 +214,7 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
             __syncthreads();
         }
         mk_spill sp;
-        sp.base = scratch + gl;
+        sp.base = spill + gl;
         sp.stride = lanes;
         auto mul = [&](const u32 s[8], fe& yn, fe& yd) {
             fe xn, xd;
@@ -303,6 +325,8 @@ extern "C" __global__ void mk_reset_kernel(mk_counters* ctr, u32* stop, u32 nsto
 
 // ------------------------------------------------------------ host launchers
 
+#include <mutex>
+
 #include "mk_launch.h"
 
 hipError_t mk_k_table_bases(hipStream_t s, u32 w, void* bases) {
@@ -324,6 +348,31 @@ hipError_t mk_k_derive(hipStream_t s, const u32* table, u32 w, const u32* key, u
     return hipGetLastError();
 }
 
+hipError_t mk_k_derive_kv(hipStream_t s, const u32* table, u32 w, const mk_key_words& key, u32 source, u64 first,
+                          u64 n, u32* out_pk, u32* out_seed) {
+    const u64 blocks = (n + 63) / 64;
+    hipLaunchKernelGGL(mk_derive_kv_kernel, dim3((u32)blocks), dim3(64), 0, s, table, w, key, source, first, n,
+                       out_pk, out_seed);
+    return hipGetLastError();
+}
+
+// Device address of `scratch` on the current device, looked up once per device (each
+// device holds its own copy at its own address) and kept for the life of the process.
+hipError_t mk_k_noalloc_scratch(int device, u32** out) {
+    static std::mutex mu;
+    static u32* ptrs[MK_MAX_DEVICES] = {};
+    if (device < 0 || device >= MK_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (ptrs[device] == nullptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        ptrs[device] = (u32*)p;
+    }
+    *out = ptrs[device];
+    return hipSuccess;
+}
+
 hipError_t mk_k_derive_seeds(hipStream_t s, const u32* table, u32 w, const u32* seeds, u32 n, u32* out_pk) {
     hipLaunchKernelGGL(mk_derive_seeds_kernel, dim3((n + 63) / 64), dim3(64), 0, s, table, w, seeds, n, out_pk);
     return hipGetLastError();
@@ -337,10 +386,10 @@ hipError_t mk_k_reset(hipStream_t s, mk_counters* ctr, u32* stop, u32 nstop, con
 
 hipError_t mk_k_search(hipStream_t s, u32 grid, const u32* table, u32 w, u32 K, const mk_search_desc* descs,
                        const mk_seg* segs, u32 nsegs, u64 total_tiles, const u64* masks, const u16* dfas,
-                       u32* scratch, mk_counters* ctr, mk_cand* cands, u32 cand_cap, u32* stop,
+                       u32* spill, mk_counters* ctr, mk_cand* cands, u32 cand_cap, u32* stop,
                        const mk_serve_state* serve, u64* next_index) {
     hipLaunchKernelGGL(mk_search_kernel, dim3(grid), dim3(64), 0, s, table, w, K, descs, segs, nsegs, total_tiles,
-                       masks, dfas, scratch, ctr, cands, cand_cap, stop, serve, next_index);
+                       masks, dfas, spill, ctr, cands, cand_cap, stop, serve, next_index);
     return hipGetLastError();
 }
 

@@ -135,7 +135,7 @@ mk_status enqueue_step(mk_server* s) {
     MK_HIPS(hipMemcpyAsync(s->d_serve, s->h_serve, sizeof(mk_serve_state), hipMemcpyHostToDevice, g->stream));
     MK_HIPS(mk_k_reset(g->stream, g->ctr, s->d_stop, MK_SERVE_MAX_SLOTS, s->d_serve));
     MK_HIPS(mk_k_search(g->stream, g->grid, g->table, g->opt.width, g->opt.batch, s->d_descs, nullptr, 0, s->step_tiles,
-                        s->d_masks, s->d_dfas, g->scratch, g->ctr, g->cands, kCandCap, s->d_stop, s->d_serve, s->d_next));
+                        s->d_masks, s->d_dfas, g->spill, g->ctr, g->cands, kCandCap, s->d_stop, s->d_serve, s->d_next));
     MK_HIPS(mk_k_confirm(g->stream, g->table, g->opt.width, s->d_descs, s->d_masks, s->d_dfas, g->ctr, g->cands, kCandCap,
                          g->hits, kHitCap));
     MK_HIPS(hipMemcpyAsync(s->h_ctr, g->ctr, sizeof(mk_counters), hipMemcpyDeviceToHost, g->stream));
