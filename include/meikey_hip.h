@@ -186,16 +186,20 @@ mk_status mk_last_timing(const mk_gpu* g, float* search_ms, float* confirm_ms);
  * launch over a fixed number of tiles dealt round-robin across the requests served in that
  * step -- the resident ones with the least attained service -- whose next key index and stop
  * flag live on the device; the step is replayed from a captured hipGraph. A request ends at its first exact hit (its lowest index found in that step),
- * when `count` candidates are spent (count 0 = unbounded), or when cancelled. The server
+ * when `count` candidates are spent (count 0 = unbounded), or when cancelled. With `overlap`,
+ * two steps are in flight on two streams, serving disjoint sets of requests: the next step
+ * starts on the SIMDs the current one frees at its end, and the host retires one step while
+ * the device runs the other (a request waits up to one more step before it starts). The server
  * uses the mk_gpu's stream and buffers: make no other call on that mk_gpu while it is open. */
 typedef struct {
     uint32_t max_slots;       /* requests resident on the device at once, 1..4096 */
     uint32_t use_graph;       /* 1: replay the captured step graph; 0: enqueue each step */
     uint64_t dfa_budget;      /* HBM bytes for the admitted requests' DFA tables */
-    uint64_t step_keys;       /* candidates per step (rounded up to whole tiles, at least one per wave) */
+    uint64_t step_keys;       /* candidates per step (rounded up to whole tiles; at least one claim unit,
+                                 128 keys per lane or a whole smaller tile, per wave) */
     uint32_t active_per_step; /* resident requests served per step, the ones with the fewest
                                  candidates searched so far (least attained service); 0 = all */
-    uint32_t pad;
+    uint32_t overlap;         /* 1: two steps in flight (disjoint requests); 0: one step at a time */
 } mk_server_options;
 
 typedef struct {

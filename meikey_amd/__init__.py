@@ -82,7 +82,7 @@ class SearchSpec(ctypes.Structure):
 
 class ServerOptions(ctypes.Structure):
     _fields_ = [("max_slots", ctypes.c_uint32), ("use_graph", ctypes.c_uint32), ("dfa_budget", ctypes.c_uint64),
-                ("step_keys", ctypes.c_uint64), ("active_per_step", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+                ("step_keys", ctypes.c_uint64), ("active_per_step", ctypes.c_uint32), ("overlap", ctypes.c_uint32)]
 
 
 class ServerResult(ctypes.Structure):
@@ -524,9 +524,9 @@ class Server:
     """Continuous batching of search requests on one Gpu (mk_server_* of the C ABI)."""
 
     def __init__(self, gpu: Gpu, max_slots: int = 1024, use_graph: bool = True, dfa_budget: int = 64 << 20,
-                 step_keys: int = 1 << 26, active_per_step: int = 64):
+                 step_keys: int = 1 << 26, active_per_step: int = 64, overlap: bool = True):
         self.gpu = gpu
-        opt = ServerOptions(max_slots, int(use_graph), dfa_budget, step_keys, active_per_step, 0)
+        opt = ServerOptions(max_slots, int(use_graph), dfa_budget, step_keys, active_per_step, int(overlap))
         h = ctypes.c_void_p()
         _check(lib().mk_server_open(gpu._h, ctypes.byref(opt), ctypes.byref(h)))
         self._h = h

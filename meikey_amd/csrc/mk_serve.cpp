@@ -14,6 +14,16 @@
 // the short requests of a heavy-tailed mix without knowing their sizes). The step -- slot-list upload, counter reset, search,
 // confirm, counter and next-index download -- is captured once as a hipGraph and
 // replayed; its kernels clear their own counters (no memset nodes).
+//
+// With `overlap`, two steps are in flight: each of two step buffers has its own stream,
+// slot list, counters, candidate and hit lists, inversion spill and graph. A call
+// launches the next step and only then waits for the previous one and retires its
+// requests, so the next step's waves start on the SIMDs the previous step's last units
+// free, and the host's retire / admit work runs while the device searches. The two steps
+// serve disjoint requests, so each request's indices are still claimed by one step at a
+// time, in order; with every claimed index searched, a request's result stays the first
+// exact hit at or after its first index. A retired slot (and its DFA memory) is reused
+// only once every step that included it has completed.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -46,6 +56,7 @@ struct pending {
 
 struct slot_state {
     bool used = false;
+    u64 last_step = 0;  // the last step that included the slot (it is reused only once that step is done)
     u64 request = 0;
     u64 first = 0, count = 0, tries = 0;
     size_t dfa_off = 0, dfa_n = 0;  // u16 units in the pool
@@ -95,6 +106,31 @@ struct pool_alloc {
 
 }  // namespace
 
+// One of the two step buffers: what a step in flight reads and writes.
+struct step_buf {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    u32* spill = nullptr;  // per-lane buffer of the batched inversion
+    bool own_spill = false;
+    mk_serve_state* d_serve = nullptr;
+    mk_counters* ctr = nullptr;
+    mk_cand* cands = nullptr;
+    mk_hit* hits = nullptr;
+    // pinned host
+    mk_serve_state* h_serve = nullptr;
+    mk_counters* h_ctr = nullptr;
+    u64* h_next = nullptr;
+    mk_hit* h_hits = nullptr;
+    u32 h_hits_cap = 0;
+    // graph of one step on this buffer
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    bool in_flight = false;
+    u64 id = 0;                                   // step number
+    std::vector<std::pair<u32, u64>> active;      // (slot, request) of the step
+};
+
 struct mk_server {
     mk_gpu* g = nullptr;
     mk_server_options opt{};
@@ -105,19 +141,20 @@ struct mk_server {
     u16* d_dfas = nullptr;
     u32* d_stop = nullptr;
     u64* d_next = nullptr;
-    mk_serve_state* d_serve = nullptr;
-    // pinned host
-    mk_serve_state* h_serve = nullptr;
-    u64* h_next = nullptr;
-    mk_counters* h_ctr = nullptr;
-    // graph of one step
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    step_buf buf[2];
+    u32 cur = 0;           // buffer of the next step
+    u64 launched = 0;      // steps launched
+    u64 completed = 0;     // steps whose results have been retired
     // host state
     std::vector<slot_state> slots;
     std::deque<pending> queue;
     std::vector<mk_server_result> done;
     pool_alloc pool;
+    struct deferred_dfa {
+        size_t off, n;
+        u64 step;
+    };
+    std::vector<deferred_dfa> deferred;  // DFA memory of retired requests, freed when their last step is done
     u64 next_id = 1;
     std::chrono::steady_clock::time_point t0;
     u64 steps = 0, keys = 0;
@@ -129,17 +166,19 @@ double now_s(const mk_server* s) {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
 }
 
-// The step's device work on the server's stream (eager, or recorded into the graph).
-mk_status enqueue_step(mk_server* s) {
+// A step's device work on buffer b, on the server's stream (eager, or recorded into b's
+// graph).
+mk_status enqueue_step(mk_server* s, u32 b) {
     mk_gpu* g = s->g;
-    MK_HIPS(hipMemcpyAsync(s->d_serve, s->h_serve, sizeof(mk_serve_state), hipMemcpyHostToDevice, g->stream));
-    MK_HIPS(mk_k_reset(g->stream, g->ctr, s->d_stop, MK_SERVE_MAX_SLOTS, s->d_serve));
-    MK_HIPS(mk_k_search(g->stream, g->grid, g->table, g->opt.width, g->opt.batch, s->d_descs, nullptr, 0, s->step_tiles,
-                        s->d_masks, s->d_dfas, g->spill, g->ctr, g->cands, kCandCap, s->d_stop, s->d_serve, s->d_next));
-    MK_HIPS(mk_k_confirm(g->stream, g->table, g->opt.width, s->d_descs, s->d_masks, s->d_dfas, g->ctr, g->cands, kCandCap,
-                         g->hits, kHitCap));
-    MK_HIPS(hipMemcpyAsync(s->h_ctr, g->ctr, sizeof(mk_counters), hipMemcpyDeviceToHost, g->stream));
-    MK_HIPS(hipMemcpyAsync(s->h_next, s->d_next, sizeof(u64) * s->opt.max_slots, hipMemcpyDeviceToHost, g->stream));
+    step_buf& B = s->buf[b];
+    MK_HIPS(hipMemcpyAsync(B.d_serve, B.h_serve, sizeof(mk_serve_state), hipMemcpyHostToDevice, B.stream));
+    MK_HIPS(mk_k_reset(B.stream, B.ctr, s->d_stop, MK_SERVE_MAX_SLOTS, B.d_serve));
+    MK_HIPS(mk_k_search(B.stream, g->grid, g->table, g->opt.width, g->opt.batch, s->d_descs, nullptr, 0, s->step_tiles,
+                        s->d_masks, s->d_dfas, B.spill, B.ctr, B.cands, kCandCap, s->d_stop, B.d_serve, s->d_next));
+    MK_HIPS(mk_k_confirm(B.stream, g->table, g->opt.width, s->d_descs, s->d_masks, s->d_dfas, B.ctr, B.cands, kCandCap,
+                         B.hits, kHitCap));
+    MK_HIPS(hipMemcpyAsync(B.h_ctr, B.ctr, sizeof(mk_counters), hipMemcpyDeviceToHost, B.stream));
+    MK_HIPS(hipMemcpyAsync(B.h_next, s->d_next, sizeof(u64) * s->opt.max_slots, hipMemcpyDeviceToHost, B.stream));
     return MK_OK;
 }
 
@@ -160,16 +199,30 @@ void finish(mk_server* s, u32 slot, int status, const mk_hit* hit, u64 tries) {
         mk_words_to_bytes(hit->pk, r.hit.pk);
     }
     s->done.push_back(r);
-    s->pool.give(st.dfa_off, st.dfa_n);
+    // a step in flight may still walk this DFA: its pool memory is released once the
+    // last step that included the slot has completed
+    s->deferred.push_back({st.dfa_off, st.dfa_n, st.last_step});
+    const u64 last = st.last_step;
     st = slot_state();
+    st.last_step = last;
 }
 
 // Move waiting requests into free slots while their DFA fits the pool.
 mk_status admit(mk_server* s) {
     mk_gpu* g = s->g;
+    for (size_t i = 0; i < s->deferred.size();) {
+        if (s->deferred[i].step <= s->completed) {
+            s->pool.give(s->deferred[i].off, s->deferred[i].n);
+            s->deferred[i] = s->deferred.back();
+            s->deferred.pop_back();
+        } else {
+            i++;
+        }
+    }
     while (!s->queue.empty()) {
+        // a free slot that no step in flight still searches
         u32 slot = 0;
-        while (slot < s->opt.max_slots && s->slots[slot].used) slot++;
+        while (slot < s->opt.max_slots && (s->slots[slot].used || s->slots[slot].last_step > s->completed)) slot++;
         if (slot == s->opt.max_slots) break;
         pending& p = s->queue.front();
         matcher_host m;
@@ -211,9 +264,61 @@ mk_status admit(mk_server* s) {
             MK_HIPS(hipMemcpy(s->d_dfas + off, m.dfa.data(), m.dfa.size() * sizeof(u16), hipMemcpyHostToDevice));
         const u64 first = p.spec.first;
         MK_HIPS(hipMemcpy(s->d_next + slot, &first, sizeof first, hipMemcpyHostToDevice));
-        s->h_next[slot] = first;  // the host copy is refreshed by every step; start it right
+        // the host snapshots are refreshed by every step; start them right
+        for (step_buf& B : s->buf) B.h_next[slot] = first;
         s->queue.pop_front();
         (void)g;
+    }
+    return MK_OK;
+}
+
+// Wait for step buffer b's step and retire its requests: each with an exact hit (its
+// lowest-index hit of the step) or with its candidate budget spent.
+mk_status retire_step(mk_server* s, u32 b) {
+    mk_gpu* g = s->g;
+    step_buf& B = s->buf[b];
+    MK_HIPS(hipEventSynchronize(B.done));
+    B.in_flight = false;
+    s->steps++;
+    s->completed = std::max(s->completed, B.id);
+    const u32 nh = std::min(B.h_ctr->n_hits, kHitCap);
+    if (nh > B.h_hits_cap) {
+        if (B.h_hits) MK_HIPS(hipHostFree(B.h_hits));
+        B.h_hits = nullptr;
+        B.h_hits_cap = 0;
+        const u32 want = std::max<u32>(nh, 4096);
+        MK_HIPS(hipHostMalloc((void**)&B.h_hits, sizeof(mk_hit) * want, hipHostMallocDefault));
+        B.h_hits_cap = want;
+    }
+    if (nh) {
+        // on the null stream: it does not wait for the step queued behind on the server's stream
+        MK_HIPS(hipMemcpy(B.h_hits, B.hits, sizeof(mk_hit) * nh, hipMemcpyDeviceToHost));
+    }
+    (void)g;
+    // lowest-index exact hit of each slot in this step. A full candidate list (overflow)
+    // drops candidates but not their slots' stop flags, so such a slot may finish on a
+    // later step's hit: every result is still an exact, confirmed match, only not always
+    // the lowest matching index of the range searched.
+    std::map<u32, mk_hit> best;
+    for (u32 i = 0; i < nh; i++) {
+        const mk_hit& h = B.h_hits[i];
+        if (h.search >= s->opt.max_slots || !s->slots[h.search].used) continue;
+        auto it = best.find(h.search);
+        if (it == best.end() || h.index < it->second.index) best[h.search] = h;
+    }
+    for (const auto& sr : B.active) {
+        const u32 slot = sr.first;
+        slot_state& ss = s->slots[slot];
+        // retired (or cancelled) since this step was launched: its hits here are stale
+        if (!ss.used || ss.request != sr.second) continue;
+        const u64 tries = B.h_next[slot] - ss.first;
+        ss.tries = tries;
+        auto it = best.find(slot);
+        if (it != best.end()) {
+            finish(s, slot, 1, &it->second, tries);
+        } else if (ss.count && tries >= ss.count) {
+            finish(s, slot, 0, nullptr, tries);
+        }
     }
     return MK_OK;
 }
@@ -229,6 +334,7 @@ void mk_server_default_options(mk_server_options* out) {
     out->dfa_budget = 64ull << 20;
     out->step_keys = 1ull << 26;
     out->active_per_step = 64;
+    out->overlap = 1;
 }
 
 mk_status mk_server_open(mk_gpu* g, const mk_server_options* opt_in, mk_server** out) {
@@ -244,7 +350,11 @@ mk_status mk_server_open(mk_gpu* g, const mk_server_options* opt_in, mk_server**
     s->g = g;
     s->opt = opt;
     const u64 tile_keys = 64ull * g->opt.batch;
-    s->step_tiles = std::max<u64>(g->grid, (opt.step_keys + tile_keys - 1) / tile_keys);
+    // The search kernel claims a serving step in units of 128 keys per lane (tiles of
+    // 256 keys per lane are two units), so a step needs one unit per wave at least.
+    const u64 units_per_tile = (g->opt.batch >= 128 && g->opt.batch % 128 == 0) ? g->opt.batch / 128 : 1;
+    const u64 min_tiles = (g->grid + units_per_tile - 1) / units_per_tile;
+    s->step_tiles = std::max<u64>(min_tiles, (opt.step_keys + tile_keys - 1) / tile_keys);
     s->slots.resize(opt.max_slots);
     const size_t dfa_units = std::max<size_t>(opt.dfa_budget / sizeof(u16), 65);
     s->pool.init(dfa_units);
@@ -255,29 +365,53 @@ mk_status mk_server_open(mk_gpu* g, const mk_server_options* opt_in, mk_server**
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_dfas, sizeof(u16) * dfa_units);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_stop, sizeof(u32) * MK_SERVE_MAX_SLOTS);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_next, sizeof(u64) * opt.max_slots);
-    if (e == hipSuccess) e = hipMalloc((void**)&s->d_serve, sizeof(mk_serve_state));
-    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_serve, sizeof(mk_serve_state), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_next, sizeof(u64) * opt.max_slots, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_ctr, sizeof(mk_counters), hipHostMallocDefault);
+    s->buf[0].stream = g->stream;
+    s->buf[0].spill = g->spill;
+    if (opt.overlap) {
+        step_buf& B = s->buf[1];
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) B.own_stream = true;
+        if (e == hipSuccess) e = hipMalloc((void**)&B.spill, g->spill_bytes);
+        if (e == hipSuccess) B.own_spill = true;
+    } else {
+        s->buf[1].stream = g->stream;
+        s->buf[1].spill = g->spill;
+    }
+    for (int b = 0; b < 2; b++) {
+        step_buf& B = s->buf[b];
+        if (e == hipSuccess) e = hipMalloc((void**)&B.d_serve, sizeof(mk_serve_state));
+        if (e == hipSuccess) e = hipMalloc((void**)&B.ctr, sizeof(mk_counters));
+        if (e == hipSuccess) e = hipMalloc((void**)&B.cands, sizeof(mk_cand) * kCandCap);
+        if (e == hipSuccess) e = hipMalloc((void**)&B.hits, sizeof(mk_hit) * kHitCap);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&B.h_serve, sizeof(mk_serve_state), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&B.h_ctr, sizeof(mk_counters), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&B.h_next, sizeof(u64) * opt.max_slots, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&B.done, hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         mk_server_close(s);
         return mk_fail(MK_ERR_DRIVER, "server allocation: %s", hipGetErrorString(e));
     }
-    memset(s->h_serve, 0, sizeof(mk_serve_state));
+    for (step_buf& B : s->buf) {
+        memset(B.h_serve, 0, sizeof(mk_serve_state));
+        memset(B.h_ctr, 0, sizeof(mk_counters));
+        memset(B.h_next, 0, sizeof(u64) * opt.max_slots);
+    }
     std::vector<u32> zeros(MK_SERVE_MAX_SLOTS, 0);
     e = hipMemcpy(s->d_stop, zeros.data(), sizeof(u32) * MK_SERVE_MAX_SLOTS, hipMemcpyHostToDevice);
-    if (e == hipSuccess && opt.use_graph) {
-        e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    for (u32 b = 0; b < 2 && e == hipSuccess && opt.use_graph; b++) {
+        step_buf& B = s->buf[b];
+        e = hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            mk_status st = enqueue_step(s);
-            hipError_t e2 = hipStreamEndCapture(g->stream, &s->graph);
+            mk_status st = enqueue_step(s, b);
+            hipError_t e2 = hipStreamEndCapture(B.stream, &B.graph);
             if (st != MK_OK) {
                 mk_server_close(s);
                 return st;
             }
             e = e2;
         }
-        if (e == hipSuccess) e = hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0);
     }
     if (e != hipSuccess) {
         mk_server_close(s);
@@ -313,9 +447,10 @@ mk_status mk_server_step(mk_server* s, uint32_t* active_out, uint32_t* queued_ou
     mk_status st = admit(s);
     if (st != MK_OK) return st;
     // least attained service: the resident requests with the fewest candidates so far
+    // (with overlap, not those the step in flight serves: the two steps take disjoint requests)
     std::vector<std::pair<u64, u32>> order;
     for (u32 i = 0; i < s->opt.max_slots; i++)
-        if (s->slots[i].used) order.emplace_back(s->slots[i].tries, i);
+        if (s->slots[i].used && s->slots[i].last_step <= s->completed) order.emplace_back(s->slots[i].tries, i);
     const size_t cap = s->opt.active_per_step ? s->opt.active_per_step : order.size();
     if (order.size() > cap) {
         std::nth_element(order.begin(), order.begin() + cap, order.end());
@@ -324,46 +459,41 @@ mk_status mk_server_step(mk_server* s, uint32_t* active_out, uint32_t* queued_ou
     std::sort(order.begin(), order.end(), [](const std::pair<u64, u32>& a, const std::pair<u64, u32>& b) {
         return a.second < b.second;
     });
-    u32 n = 0;
-    for (const auto& o : order) s->h_serve->slots[n++] = o.second;
-    s->h_serve->n_active = n;
+    const u32 n = (u32)order.size();
     if (active_out) *active_out = n;
     if (queued_out) *queued_out = (u32)s->queue.size();
-    if (n == 0) return MK_OK;
-    if (s->exec) {
-        MK_HIPS(hipGraphLaunch(s->exec, g->stream));
-    } else {
-        st = enqueue_step(s);
-        if (st != MK_OK) return st;
-    }
-    MK_HIPS(hipStreamSynchronize(g->stream));
-    s->steps++;
-    const u32 nh = std::min(s->h_ctr->n_hits, kHitCap);
-    std::vector<mk_hit> hits(nh);
-    if (nh) {
-        MK_HIPS(hipMemcpy(hits.data(), g->hits, sizeof(mk_hit) * nh, hipMemcpyDeviceToHost));
-    }
-    // lowest-index exact hit of each slot in this step. A full candidate list (overflow)
-    // drops candidates but not their slots' stop flags, so such a slot may finish on a
-    // later step's hit: every result is still an exact, confirmed match, only not always
-    // the lowest matching index of the range searched.
-    std::map<u32, mk_hit> best;
-    for (const mk_hit& h : hits) {
-        if (h.search >= s->opt.max_slots || !s->slots[h.search].used) continue;
-        auto it = best.find(h.search);
-        if (it == best.end() || h.index < it->second.index) best[h.search] = h;
-    }
-    for (u32 i = 0; i < n; i++) {
-        const u32 slot = s->h_serve->slots[i];
-        const u64 tries = s->h_next[slot] - s->slots[slot].first;
-        s->slots[slot].tries = tries;
-        auto it = best.find(slot);
-        if (it != best.end()) {
-            finish(s, slot, 1, &it->second, tries);
-        } else if (s->slots[slot].count && tries >= s->slots[slot].count) {
-            finish(s, slot, 0, nullptr, tries);
+    const u32 prev = s->cur ^ 1u;
+    if (n > 0) {
+        // buffer `cur` was last used two steps ago; retire_step has waited for it
+        step_buf& B = s->buf[s->cur];
+        if (B.in_flight) {
+            st = retire_step(s, s->cur);
+            if (st != MK_OK) return st;
         }
+        B.active.clear();
+        for (u32 i = 0; i < n; i++) {
+            const u32 slot = order[i].second;
+            B.h_serve->slots[i] = slot;
+            B.active.emplace_back(slot, s->slots[slot].request);
+        }
+        B.h_serve->n_active = n;
+        B.id = ++s->launched;
+        for (u32 i = 0; i < n; i++) s->slots[order[i].second].last_step = B.id;
+        if (B.exec) {
+            MK_HIPS(hipGraphLaunch(B.exec, B.stream));
+        } else {
+            st = enqueue_step(s, s->cur);
+            if (st != MK_OK) return st;
+        }
+        MK_HIPS(hipEventRecord(B.done, B.stream));
+        B.in_flight = true;
+        if (!s->opt.overlap) return retire_step(s, s->cur);
+        s->cur ^= 1u;
     }
+    // the previous step ran while this one was queued: retire its requests now (with no
+    // new step, the one just before)
+    if (s->buf[prev].in_flight) return retire_step(s, prev);
+    if (n == 0 && s->buf[s->cur].in_flight) return retire_step(s, s->cur);
     return MK_OK;
 }
 
@@ -392,7 +522,7 @@ mk_status mk_server_cancel(mk_server* s, uint64_t request) {
         }
     for (u32 i = 0; i < s->opt.max_slots; i++)
         if (s->slots[i].used && s->slots[i].request == request) {
-            finish(s, i, -1, nullptr, s->h_next[i] - s->slots[i].first);
+            finish(s, i, -1, nullptr, s->slots[i].tries);  // as of the last retired step
             return MK_OK;
         }
     return mk_fail(MK_ERR_INVALID, "no request %llu", (unsigned long long)request);
@@ -409,17 +539,28 @@ mk_status mk_server_close(mk_server* s) {
     if (!s) return MK_OK;
     (void)hipSetDevice(s->g->device);
     (void)hipStreamSynchronize(s->g->stream);
-    if (s->exec) (void)hipGraphExecDestroy(s->exec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
+    for (step_buf& B : s->buf)
+        if (B.own_stream) (void)hipStreamSynchronize(B.stream);
+    for (step_buf& B : s->buf) {
+        if (B.own_stream) (void)hipStreamDestroy(B.stream);
+        if (B.own_spill) (void)hipFree(B.spill);
+        if (B.exec) (void)hipGraphExecDestroy(B.exec);
+        if (B.graph) (void)hipGraphDestroy(B.graph);
+        if (B.done) (void)hipEventDestroy(B.done);
+        if (B.d_serve) (void)hipFree(B.d_serve);
+        if (B.ctr) (void)hipFree(B.ctr);
+        if (B.cands) (void)hipFree(B.cands);
+        if (B.hits) (void)hipFree(B.hits);
+        if (B.h_serve) (void)hipHostFree(B.h_serve);
+        if (B.h_ctr) (void)hipHostFree(B.h_ctr);
+        if (B.h_next) (void)hipHostFree(B.h_next);
+        if (B.h_hits) (void)hipHostFree(B.h_hits);
+    }
     if (s->d_descs) (void)hipFree(s->d_descs);
     if (s->d_masks) (void)hipFree(s->d_masks);
     if (s->d_dfas) (void)hipFree(s->d_dfas);
     if (s->d_stop) (void)hipFree(s->d_stop);
     if (s->d_next) (void)hipFree(s->d_next);
-    if (s->d_serve) (void)hipFree(s->d_serve);
-    if (s->h_serve) (void)hipHostFree(s->h_serve);
-    if (s->h_next) (void)hipHostFree(s->h_next);
-    if (s->h_ctr) (void)hipHostFree(s->h_ctr);
     delete s;
     return MK_OK;
 }

@@ -552,12 +552,13 @@ def _serve_specs(n, seed):
     return out
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_server_results_alone_and_in_the_mix(gpu, oracle, use_graph):
+@pytest.mark.parametrize("use_graph,overlap", [(True, True), (False, True), (True, False)])
+def test_server_results_alone_and_in_the_mix(gpu, oracle, use_graph, overlap):
     """Continuous batching: requests submitted while others run; each result equals the
-    result of the same request served alone, and the oracle confirms it."""
+    result of the same request served alone, and the oracle confirms it. With overlap, two
+    steps serving disjoint requests are in flight on two streams."""
     specs = _serve_specs(24, 7)
-    with M.Server(gpu, max_slots=8, use_graph=use_graph, step_keys=1 << 22) as srv:
+    with M.Server(gpu, max_slots=8, use_graph=use_graph, step_keys=1 << 22, overlap=overlap) as srv:
         ids = {}
         results = {}
         i = 0
@@ -577,7 +578,7 @@ def test_server_results_alone_and_in_the_mix(gpu, oracle, use_graph):
         _served_checks(oracle, s, results[k])
     # alone
     for k in (0, 5, 11, 23):
-        with M.Server(gpu, max_slots=4, use_graph=use_graph, step_keys=1 << 22) as srv:
+        with M.Server(gpu, max_slots=4, use_graph=use_graph, step_keys=1 << 22, overlap=overlap) as srv:
             rid = srv.submit(specs[k])
             got = []
             while not got:
