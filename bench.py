@@ -28,7 +28,9 @@ METRIC = "secure-mode keys/s on 1 MI355X (single vs 64-search batched); p99 time
 STEP_KEYS = 1 << 29
 ctypes_u32, ctypes_u64, ctypes_buf = ctypes.c_uint32, ctypes.c_uint64, ctypes.create_string_buffer
 # MI355X integer VALU peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, one 32x32->64
-# multiply-add (v_mad_u64_u32) per lane per clock.
+# multiply-add (v_mad_u64_u32) per lane per clock. Measured: v_mad_u64_u32 issues one
+# wave64 instruction per SIMD per 4 clocks and never pairs with another VALU instruction;
+# only plain 32-bit operations dual-issue (meikey_amd/mk_valu_probe).
 PEAK_MUL = 256 * 4 * 16 * 2.4e9
 
 
@@ -101,6 +103,10 @@ def main():
     achieved = STEP_KEYS * mults * 64 / (avg_ms * 1e-3)
     achieved_run = STEP_KEYS * run_mults * 64 / (avg_ms * 1e-3)
     traffic = measured_traffic(args.width, args.batch, args.blocks)
+    issue = (profiled(args.width, args.batch, args.blocks) or {}).get("valu_issue") or {}
+    # the same peak at the clock the box sustains through a full-chip launch (the counter
+    # run's GRBM_GUI_ACTIVE over its time)
+    peak_clock = 256 * 4 * 16 * issue["clock_ghz"] * 1e9 if issue.get("clock_ghz") else None
     out = {
         "metric": METRIC, "value": rate, "unit": "keys/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
@@ -116,13 +122,20 @@ def main():
                      "algorithmic": f"{mults:.2f} field mults/key (W=16, K=256) x 64 32x32-bit products",
                      "achieved_run_mults": achieved_run / 1e12,
                      "run_mults_per_key": run_mults,
+                     # the multiplications this run does (W and K as run), against the
+                     # same 39.3 T peak and against the peak at the sustained clock
+                     "frac_run_mults": achieved_run / PEAK_MUL,
+                     "peak_at_sustained_clock": peak_clock / 1e12 if peak_clock else None,
+                     "sustained_clock_ghz": issue.get("clock_ghz"),
+                     "frac_at_sustained_clock": achieved / peak_clock if peak_clock else None,
+                     "frac_run_mults_at_sustained_clock": achieved_run / peak_clock if peak_clock else None,
                      # integer VALU is the bound (no dense contraction for MFMA); the HBM
                      # side of the same launches, from the committed traffic counters
                      "hbm": ({"achieved": traffic / (avg_ms * 1e-3) / 1e12, "peak": 8.0, "unit": "TB/s",
                               "frac": traffic / (avg_ms * 1e-3) / 8e12} if traffic else None),
                      # the bound itself: how often the SIMDs issue a VALU instruction, from
                      # the committed SQ counter run of the same launch
-                     "valu_issue": (profiled(args.width, args.batch, args.blocks) or {}).get("valu_issue")},
+                     "valu_issue": issue or None},
     }
     if not args.no_batch:
         out["batched"] = batched_leg(gpu, rate)

@@ -4,6 +4,7 @@ libmeikey_hip.so      hipcc --offload-arch=gfx950: kernels + C ABI (include/meik
                       host re-derivation of emitted keys compiled by g++
 libmeikey_hostcheck.so  g++: the same arithmetic headers compiled for the host, test hooks only
 mk_example            gcc: a C client of the ABI (open, pattern, search, OpenSSH line)
+mk_valu_probe         hipcc: issue rates of the VALU instructions the search kernel is built from
 """
 from __future__ import annotations
 
@@ -16,6 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmeikey_hip.so")
 HOSTCHECK = os.path.join(HERE, "libmeikey_hostcheck.so")
 EXAMPLE = os.path.join(HERE, "mk_example")
+PROBE = os.path.join(HERE, "mk_valu_probe")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # gfx950 only; packed-FP32 instructions stay off (the kernels are integer code anyway).
@@ -73,10 +75,18 @@ def build_example(out: str = EXAMPLE) -> str:
     return out
 
 
+def build_probe(out: str = PROBE) -> str:
+    """The VALU issue-rate probe (csrc/mk_valu_probe.hip), a stand-alone program."""
+    flags = [f for f in HIP_FLAGS if f not in ("-shared", "-fPIC")]
+    _run([HIPCC, *flags, "-o", out, os.path.join(CSRC, "mk_valu_probe.hip")])
+    return out
+
+
 def build_all() -> None:
     build_hostcheck()
     build_hip()
     build_example()
+    build_probe()
 
 
 if __name__ == "__main__":
