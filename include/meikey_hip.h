@@ -61,6 +61,9 @@ typedef struct {
     uint32_t batch;  /* keys per thread per tile, 1..256 (one inversion per `batch` keys per lane) */
     uint32_t limbs;  /* MK_LIMBS_R25 */
     uint32_t blocks; /* resident waves per SIMD the grid is sized for, 1..8 (reference: blocks per SM) */
+    uint32_t lockstep; /* 0: one wave per workgroup; 1: the waves of a CU form one workgroup and add each
+                          comb row together, which keeps the CU's address translation on one or two rows
+                          of the table (blocks 1..2); 2: the same with workgroups of one wave per SIMD */
 } mk_options;
 
 /* Device status (gpu.rs:1251-1326 status rows). */

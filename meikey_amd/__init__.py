@@ -59,7 +59,7 @@ class MeikeyError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("batch", ctypes.c_uint32), ("limbs", ctypes.c_uint32),
-                ("blocks", ctypes.c_uint32)]
+                ("blocks", ctypes.c_uint32), ("lockstep", ctypes.c_uint32)]
 
 
 class Info(ctypes.Structure):
@@ -246,6 +246,13 @@ def fastest_staged(ordinal: int = 0):
     return best, rows
 
 
+def default_options() -> Options:
+    """mk_default_options: the library's default setting."""
+    o = Options()
+    lib().mk_default_options(ctypes.byref(o))
+    return o
+
+
 def device_count() -> int:
     n = ctypes.c_int(0)
     _check(lib().mk_device_count(ctypes.byref(n)))
@@ -381,8 +388,9 @@ class Launch:
 class Gpu:
     """One device with its comb table (the reference's Gpu + Kernel)."""
 
-    def __init__(self, ordinal: int = 0, width: int = 24, batch: int = 256, blocks: int = 2, limbs: int = LIMBS_R25):
-        self.options = Options(width, batch, limbs, blocks)
+    def __init__(self, ordinal: int = 0, width: int = 24, batch: int = 256, blocks: int = 2, limbs: int = LIMBS_R25,
+                 lockstep: int = 0):
+        self.options = Options(width, batch, limbs, blocks, lockstep)
         h = ctypes.c_void_p()
         _check(lib().mk_open(ordinal, ctypes.byref(self.options), ctypes.byref(h)))
         self._h = h

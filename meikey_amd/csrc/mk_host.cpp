@@ -83,7 +83,7 @@ mk_status mk_run_step(mk_gpu* g, const mk_search_desc* descs, const u64* masks, 
     MK_HIP(mk_k_reset(g->stream, g->ctr, stop, nstop, nullptr));
     MK_HIP(hipEventRecord(g->ev[0], g->stream));
     if (total_tiles > 0)
-        MK_HIP(mk_k_search(g->stream, g->grid, g->table, g->opt.width, g->opt.batch, descs, segs, nsegs, total_tiles,
+        MK_HIP(mk_k_search(g->stream, g->grid, g->lock_waves, g->table, g->opt.width, g->opt.batch, descs, segs, nsegs, total_tiles,
                            masks, dfas, g->spill, g->ctr, g->cands, kCandCap, stop, nullptr, nullptr));
     MK_HIP(hipEventRecord(g->ev[1], g->stream));
     if (total_tiles > 0)
@@ -231,6 +231,7 @@ void mk_default_options(mk_options* out) {
     out->batch = 256;
     out->limbs = MK_LIMBS_R25;
     out->blocks = 2;
+    out->lockstep = 0;
 }
 
 mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
@@ -246,6 +247,10 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
                     "batch %u, blocks %u)",
                     MK_MIN_W, MK_MAX_W, kMaxBatch, opt.width, opt.batch, opt.blocks);
     if (opt.limbs != MK_LIMBS_R25) return fail(MK_ERR_OUT_OF_RANGE, "limb layout %u is not available", opt.limbs);
+    if (opt.lockstep > 2 || (opt.lockstep == 1 && opt.blocks > 2))
+        return fail(MK_ERR_OUT_OF_RANGE,
+                    "lockstep is 0, 1 (one workgroup per CU, 1 or 2 waves per SIMD) or 2 (workgroups of one wave per "
+                    "SIMD) (lockstep %u, blocks %u)", opt.lockstep, opt.blocks);
     int count = 0;
     mk_status st = mk_device_count(&count);
     if (st != MK_OK) return st;
@@ -274,6 +279,8 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
     g->cus = (u32)prop.multiProcessorCount;
     g->hbm = prop.totalGlobalMem;
     g->grid = g->cus * 4 * opt.blocks;
+    // lockstep 1: one workgroup per CU; 2: workgroups of one wave per SIMD
+    g->lock_waves = opt.lockstep == 1 ? 4 * opt.blocks : opt.lockstep == 2 ? 4 : 0;
     g->lanes = (u64)g->grid * 64;
     g->rows = rows_of(opt.width);
     g->cols = cols_of(opt.width);

@@ -48,6 +48,7 @@ struct mk_gpu {
     hipStream_t stream = nullptr;
     mk_options opt{};
     u32 cus = 0, grid = 0, rows = 0, cols = 0;
+    u32 lock_waves = 0;  // waves per workgroup of the lockstep search kernel (0: one-wave workgroups)
     u64 lanes = 0, hbm = 0;
     char name[64] = {};
     u32* table = nullptr;

@@ -118,157 +118,285 @@ __device__ static u64 mk_uniform64(u64 v) {
 // Largest DFA that fits the 16 KB staging buffers (states x 65 x 2 bytes).
 #define MK_LDS_DFA_STATES 126
 
+// Orders this wave's LDS writes before its later LDS reads by other lanes (the per-wave
+// buffers are shared by no other wave, so no workgroup barrier is needed).
+__device__ __forceinline__ void mk_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The kernel arguments of both search kernels.
+struct mk_search_args {
+    const u32* __restrict__ table;
+    u32 w, K;
+    const mk_search_desc* __restrict__ descs;
+    const mk_seg* __restrict__ segs;
+    u32 nsegs;
+    u64 total_tiles;
+    const u64* __restrict__ masks;
+    const u16* __restrict__ dfas;
+    u32* __restrict__ spill;
+    mk_counters* __restrict__ ctr;
+    mk_cand* __restrict__ cands;
+    u32 cand_cap;
+    u32* __restrict__ stop;
+    const mk_serve_state* __restrict__ serve;
+    u64* __restrict__ next_index;
+};
+
+// One wave's LDS and identity.
+struct mk_wave_ctx {
+    u32 lane;
+    u64 gl;     // global lane number (spill column)
+    u64 lanes;  // lanes of the grid (spill row stride)
+    u32 cached; // search whose bitap table is in lmask
+    u64* lmask;
+    uint4* stage;
+    mk_sha_mid* smid;
+};
+
+// Tail balancing: the last tiles (one per wave of the grid) are cut into U units of K / U
+// keys per lane, claimed one by one from a second counter, so the waves finish within a
+// fraction of a tile of each other. Units keep at least 64 keys per lane (the inversion
+// is shared by a unit's keys). A serving step of one tile per wave is therefore claimed
+// entirely in units; there they are 128 keys, which balance the step's mix of matchers
+// at half the inversions of 64-key units (0.5% more served keys/s; whole tiles serve 5%
+// fewer).
+__device__ __forceinline__ u32 mk_units_per_tile(u32 K, bool serving) {
+    return serving ? ((K >= 128 && K % 128 == 0) ? K / 128 : 1) : ((K >= 128 && K % 64 == 0) ? K / 64 : 1);
+}
+
+// One claim of one wave: tile `tile` (sub-unit `sub` of a tail tile) with Keff keys per
+// lane; nth is the claim's number in claim order (serving deals claims round-robin over
+// the active requests). Returns false, having searched nothing, when the claim's search
+// has stopped; otherwise the comb multiply calls row_sync() Keff x (rows - 2) times.
+template <class R>
+__device__ __forceinline__ bool mk_search_claim(const mk_search_args& a, mk_wave_ctx& cx, u64 tile, u64 sub,
+                                                u32 Keff, u64 nth, R row_sync) {
+    const u32 lane = cx.lane;
+    u32 search;
+    u64 tile_first;
+    if (a.serve != nullptr) {
+        // serving: round-robin over the active slots, indices from the slot's counter
+        search = __builtin_amdgcn_readfirstlane(a.serve->slots[nth % a.serve->n_active]);
+        // the decision is taken once for the whole wave (readfirstlane), so the lockstep
+        // kernel's barrier count never differs between its lanes
+        if (__builtin_amdgcn_readfirstlane(__atomic_load_n(&a.stop[search], __ATOMIC_RELAXED)) != 0) return false;
+        u64 f = 0;
+        if (lane == 0) f = atomicAdd((unsigned long long*)&a.next_index[search], 64ull * Keff);
+        tile_first = mk_uniform64(f);
+    } else {
+        // segment holding this tile (binary search; segments are few and in order)
+        u32 lo = 0, hi = a.nsegs;
+        while (hi - lo > 1) {
+            const u32 mid = (lo + hi) >> 1;
+            if (a.segs[mid].tile_begin <= tile) lo = mid;
+            else hi = mid;
+        }
+        const mk_seg sg = a.segs[lo];
+        search = sg.search;
+        tile_first = sg.first + (tile - sg.tile_begin) * 64ull * a.K + sub * 64ull * Keff;
+    }
+    const mk_search_desc& dsc = a.descs[search];
+    // A serving unit has already advanced its request's next index, so it is searched
+    // even if the request stopped meanwhile: skipping it would leave a hole of
+    // unsearched indices that still count as tries (the stop flag was checked before
+    // the claim, and the request may go on after a candidate that the exact check
+    // rejects).
+    if (a.serve == nullptr && dsc.stop_on_hit &&
+        __builtin_amdgcn_readfirstlane(__atomic_load_n(&a.stop[search], __ATOMIC_RELAXED)) != 0)
+        return false;
+    if (search != cx.cached) {
+        mk_wave_sync();
+        cx.lmask[lane] = a.masks[(u64)dsc.mask_off + lane];
+        mk_wave_sync();
+        cx.cached = search;
+    }
+    mk_matcher mt;
+    mt.mode = dsc.mode;
+    mt.len = dsc.len;
+    mt.mask = cx.lmask;
+    mt.dfa = a.dfas + dsc.dfa_off;
+    u32 key[8];
+    MK_UNROLL
+    for (int i = 0; i < 8; i++) key[i] = dsc.key[i];
+    const u64 first = tile_first + (u64)lane * Keff;
+    const u32 stop_on_hit = dsc.stop_on_hit;
+    if (dsc.source == 1) {
+        // the counter seeds' SHA-512 mid-state, the same for the whole wave
+        mk_sha_mid m;
+        mk_sha_mid_init(key, m);
+        mk_wave_sync();
+        if (lane == 0) *cx.smid = m;
+        mk_wave_sync();
+    }
+    mk_spill sp;
+    sp.base = a.spill + cx.gl;
+    sp.stride = cx.lanes;
+    uint4* stage = cx.stage;
+    auto mul = [&](const u32 s[8], fe& yn, fe& yd) {
+        fe xn, xd;
+        mk_mul_base_frac_staged(a.table, a.w, s, stage, lane, yn, yd, xn, xd, row_sync);
+    };
+    // During the back-sweep the staging buffers are idle: a DFA of up to
+    // MK_LDS_DFA_STATES states is copied into them so its 43 dependent lookups per
+    // key hit LDS instead of L2.
+    auto pre_sweep = [&] {
+        if (mt.mode == MK_MODE_REGEX && mt.len <= MK_LDS_DFA_STATES) {
+            u16* ldfa = (u16*)stage;
+            const u16* src = a.dfas + dsc.dfa_off;
+            for (u32 i = lane; i < mt.len * 65; i += 64) ldfa[i] = src[i];
+            mk_wave_sync();
+            mt.dfa = ldfa;
+        }
+    };
+    // volatile: loaded where each key's hash needs it, not kept in registers
+    const mk_sha_mid* smid = cx.smid;
+    auto mid = [&]() {
+        const volatile mk_sha_mid* v = smid;
+        mk_sha_mid m;
+        MK_UNROLL
+        for (int i = 0; i < 8; i++) m.v[i] = v->v[i];
+        m.w2 = v->w2;
+        m.w16 = v->w16;
+        m.w17 = v->w17;
+        return m;
+    };
+    mk_counters* ctr = a.ctr;
+    mk_cand* cands = a.cands;
+    const u32 cand_cap = a.cand_cap;
+    u32* stop = a.stop;
+    mk_lane_tile(Keff, key, dsc.source, first, sp, mul, pre_sweep, [&](u32 jj, const u32 pk[8]) {
+        const bool cand = mk_pk_match(pk, mt, true);
+        const u64 bal = __ballot(cand);
+        if (bal == 0) return;
+        const u32 leader = (u32)__builtin_ctzll(bal);
+        const u32 below = __builtin_amdgcn_mbcnt_hi((u32)(bal >> 32), __builtin_amdgcn_mbcnt_lo((u32)bal, 0u));
+        u32 base = 0;
+        if (lane == leader) {
+            base = atomicAdd(&ctr->n_cands, (u32)__builtin_popcountll(bal));
+            if (stop_on_hit) atomicExch(&stop[search], 1u);
+        }
+        base = __shfl(base, (int)leader);
+        if (cand) {
+            const u32 slot = base + below;
+            if (slot < cand_cap) {
+                cands[slot].search = search;
+                cands[slot].pad = 0;
+                cands[slot].index = first + jj;
+            } else {
+                atomicOr(&ctr->overflow, 1u);
+            }
+        }
+    }, mid);
+    return true;
+}
+
+// The search kernel: one wave per workgroup, each wave claims its own tiles.
 extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
-    mk_search_kernel(const u32* __restrict__ table, u32 w, u32 K, const mk_search_desc* __restrict__ descs,
-                     const mk_seg* __restrict__ segs, u32 nsegs, u64 total_tiles, const u64* __restrict__ masks,
-                     const u16* __restrict__ dfas, u32* __restrict__ spill, mk_counters* __restrict__ ctr,
-                     mk_cand* __restrict__ cands, u32 cand_cap, u32* __restrict__ stop,
-                     const mk_serve_state* __restrict__ serve, u64* __restrict__ next_index) {
+    mk_search_kernel(mk_search_args a) {
     __shared__ u64 lmask[64];
     __shared__ uint4 stage[2 * 8 * 64];
     __shared__ mk_sha_mid smid;
-    const u32 lane = threadIdx.x;
-    const u64 lanes = (u64)gridDim.x * 64;
-    const u64 gl = (u64)blockIdx.x * 64 + lane;
-    u32 cached = 0xffffffffu;
-    // Tail balancing: the last `gridDim.x` tiles are cut into U units of K / U keys per
-    // lane, claimed one by one from a second counter, so the waves finish within a
-    // fraction of a tile of each other. Units keep at least 64 keys per lane (the
-    // inversion is shared by a unit's keys). A serving step of one tile per wave is
-    // therefore claimed entirely in units; there they are 128 keys, which balance the
-    // step's mix of matchers at half the inversions of 64-key units (0.5% more served
-    // keys/s; whole tiles serve 5% fewer).
-    const u32 U = serve != nullptr ? ((K >= 128 && K % 128 == 0) ? K / 128 : 1)
-                                   : ((K >= 128 && K % 64 == 0) ? K / 64 : 1);
+    mk_wave_ctx cx;
+    cx.lane = threadIdx.x;
+    cx.lanes = (u64)gridDim.x * 64;
+    cx.gl = (u64)blockIdx.x * 64 + cx.lane;
+    cx.cached = 0xffffffffu;
+    cx.lmask = lmask;
+    cx.stage = stage;
+    cx.smid = &smid;
+    const u32 K = a.K;
+    const u32 U = mk_units_per_tile(K, a.serve != nullptr);
+    const u64 total_tiles = a.total_tiles;
     const u64 tail = U > 1 ? (total_tiles < gridDim.x ? total_tiles : gridDim.x) : 0;
     const u64 big_tiles = total_tiles - tail;
     for (;;) {
-        u64 tile = 0, sub = 0;
-        u32 Keff = K;
-        {
-            u64 claim = 0;
-            if (lane == 0) {
-                const u64 t = atomicAdd((unsigned long long*)&ctr->queue, 1ull);
-                claim = t < big_tiles ? t * U : big_tiles * U + atomicAdd((unsigned long long*)&ctr->queue2, 1ull);
-            }
-            claim = mk_uniform64(claim);
-            tile = claim / U;
-            sub = claim % U;
-            if (tile >= big_tiles) Keff = K / U;
+        u64 claim = 0;
+        if (cx.lane == 0) {
+            const u64 t = atomicAdd((unsigned long long*)&a.ctr->queue, 1ull);
+            claim = t < big_tiles ? t * U : big_tiles * U + atomicAdd((unsigned long long*)&a.ctr->queue2, 1ull);
         }
+        claim = mk_uniform64(claim);
+        const u64 tile = claim / U, sub = claim % U;
+        const u32 Keff = tile >= big_tiles ? K / U : K;
         if (tile >= total_tiles) break;
-        u32 search;
-        u64 tile_first;
-        if (serve != nullptr) {
-            // serving: round-robin over the active slots, indices from the slot's counter
-            const u32 n_active = serve->n_active;
-            if (n_active == 0) break;
-            // claim number: whole tiles first, then the units of the tail
-            const u64 nth = tile < big_tiles ? tile : big_tiles + (tile - big_tiles) * U + sub;
-            search = serve->slots[nth % n_active];
-            if (__atomic_load_n(&stop[search], __ATOMIC_RELAXED) != 0) continue;
-            u64 f = 0;
-            if (lane == 0) f = atomicAdd((unsigned long long*)&next_index[search], 64ull * Keff);
-            tile_first = mk_uniform64(f);
-        } else {
-            // segment holding this tile (binary search; segments are few and in order)
-            u32 lo = 0, hi = nsegs;
-            while (hi - lo > 1) {
-                const u32 mid = (lo + hi) >> 1;
-                if (segs[mid].tile_begin <= tile) lo = mid;
-                else hi = mid;
-            }
-            const mk_seg sg = segs[lo];
-            search = sg.search;
-            tile_first = sg.first + (tile - sg.tile_begin) * 64ull * K + sub * 64ull * Keff;
-        }
-        const mk_search_desc& dsc = descs[search];
-        // A serving unit has already advanced its request's next index, so it is searched
-        // even if the request stopped meanwhile: skipping it would leave a hole of
-        // unsearched indices that still count as tries (the stop flag was checked before
-        // the claim, and the request may go on after a candidate that the exact check
-        // rejects).
-        if (serve == nullptr && dsc.stop_on_hit && __atomic_load_n(&stop[search], __ATOMIC_RELAXED) != 0) continue;
-        if (search != cached) {
-            __syncthreads();
-            lmask[lane] = masks[(u64)dsc.mask_off + lane];
-            __syncthreads();
-            cached = search;
-        }
-        mk_matcher mt;
-        mt.mode = dsc.mode;
-        mt.len = dsc.len;
-        mt.mask = lmask;
-        mt.dfa = dfas + dsc.dfa_off;
-        u32 key[8];
-        MK_UNROLL
-        for (int i = 0; i < 8; i++) key[i] = dsc.key[i];
-        const u64 first = tile_first + (u64)lane * Keff;
-        const u32 stop_on_hit = dsc.stop_on_hit;
-        if (dsc.source == 1) {
-            // the counter seeds' SHA-512 mid-state, the same for the whole wave
-            mk_sha_mid m;
-            mk_sha_mid_init(key, m);
-            __syncthreads();
-            if (lane == 0) smid = m;
-            __syncthreads();
-        }
-        mk_spill sp;
-        sp.base = spill + gl;
-        sp.stride = lanes;
-        auto mul = [&](const u32 s[8], fe& yn, fe& yd) {
-            fe xn, xd;
-            mk_mul_base_frac_staged(table, w, s, stage, lane, yn, yd, xn, xd);
-        };
-        // During the back-sweep the staging buffers are idle: a DFA of up to
-        // MK_LDS_DFA_STATES states is copied into them so its 43 dependent lookups per
-        // key hit LDS instead of L2.
-        auto pre_sweep = [&] {
-            if (mt.mode == MK_MODE_REGEX && mt.len <= MK_LDS_DFA_STATES) {
-                u16* ldfa = (u16*)stage;
-                const u16* src = dfas + dsc.dfa_off;
-                for (u32 i = lane; i < mt.len * 65; i += 64) ldfa[i] = src[i];
-                __syncthreads();
-                mt.dfa = ldfa;
-            }
-        };
-        // volatile: loaded where each key's hash needs it, not kept in registers
-        auto mid = [&]() {
-            const volatile mk_sha_mid* v = &smid;
-            mk_sha_mid m;
-            MK_UNROLL
-            for (int i = 0; i < 8; i++) m.v[i] = v->v[i];
-            m.w2 = v->w2;
-            m.w16 = v->w16;
-            m.w17 = v->w17;
-            return m;
-        };
-        mk_lane_tile(Keff, key, dsc.source, first, sp, mul, pre_sweep, [&](u32 jj, const u32 pk[8]) {
-            const bool cand = mk_pk_match(pk, mt, true);
-            const u64 bal = __ballot(cand);
-            if (bal == 0) return;
-            const u32 leader = (u32)__builtin_ctzll(bal);
-            const u32 below = __builtin_amdgcn_mbcnt_hi((u32)(bal >> 32), __builtin_amdgcn_mbcnt_lo((u32)bal, 0u));
-            u32 base = 0;
-            if (lane == leader) {
-                base = atomicAdd(&ctr->n_cands, (u32)__builtin_popcountll(bal));
-                if (stop_on_hit) atomicExch(&stop[search], 1u);
-            }
-            base = __shfl(base, (int)leader);
-            if (cand) {
-                const u32 slot = base + below;
-                if (slot < cand_cap) {
-                    cands[slot].search = search;
-                    cands[slot].pad = 0;
-                    cands[slot].index = first + jj;
-                } else {
-                    atomicOr(&ctr->overflow, 1u);
-                }
-            }
-        }, mid);
+        if (a.serve != nullptr && a.serve->n_active == 0) break;
+        // claim number: whole tiles first, then the units of the tail
+        const u64 nth = tile < big_tiles ? tile : big_tiles + (tile - big_tiles) * U + sub;
+        mk_search_claim(a, cx, tile, sub, Keff, nth, [] {});
     }
 }
 
+// The lockstep search kernel: one workgroup of 4 x (waves per SIMD) waves per CU. The
+// workgroup claims a tile (or tail unit) for each of its waves at once, all with the same
+// number of keys per lane, and its waves pass a barrier at every comb row, so the waves of
+// a CU gather from one or two rows of the table at a time instead of all of them: the
+// CU's address-translation cache then covers what it reads even for tables far larger
+// than its reach. A wave whose claim has nothing to search passes the same barriers
+// (Keff x (rows - 2) of them) without work, so every wave of the workgroup executes the
+// same barrier sequence. Every branch that decides how many barriers a wave passes
+// (the claim, the end of the queue, an empty serving step) reads values that are equal
+// for the whole workgroup.
+template <u32 NW>
+__device__ __forceinline__ void mk_search_lock(const mk_search_args& a) {
+    __shared__ u64 lmask[NW][64];
+    __shared__ uint4 stage[NW][2 * 8 * 64];
+    __shared__ mk_sha_mid smid[NW];
+    __shared__ u64 wg_claim[2];
+    const u32 nw = NW;  // waves of this workgroup
+    const u32 wave = threadIdx.x / 64;
+    mk_wave_ctx cx;
+    cx.lane = threadIdx.x % 64;
+    cx.lanes = (u64)gridDim.x * blockDim.x;
+    cx.gl = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    cx.cached = 0xffffffffu;
+    cx.lmask = lmask[wave];
+    cx.stage = stage[wave];
+    cx.smid = &smid[wave];
+    const u32 K = a.K;
+    const u32 U = mk_units_per_tile(K, a.serve != nullptr);
+    const u64 total_tiles = a.total_tiles;
+    const u64 waves = (u64)gridDim.x * nw;
+    // whole tiles come in groups of nw, so a group never straddles the tail
+    u64 big_tiles = total_tiles - (U > 1 ? (total_tiles < waves ? total_tiles : waves) : 0);
+    if (U > 1) big_tiles -= big_tiles % nw;
+    const u64 total_claims = big_tiles + (total_tiles - big_tiles) * U;  // whole tiles, then units
+    const u32 rows = mk_rows(a.w);
+    for (u32 round = 0;; round++) {
+        if (threadIdx.x == 0) {
+            const u64 t = atomicAdd((unsigned long long*)&a.ctr->queue, (unsigned long long)nw);
+            wg_claim[round & 1] =
+                t < big_tiles ? t : big_tiles + atomicAdd((unsigned long long*)&a.ctr->queue2, (unsigned long long)nw);
+        }
+        __syncthreads();
+        const u64 c0 = mk_uniform64(wg_claim[round & 1]);  // first claim of the group, the same for all waves
+        if (c0 >= total_claims) break;
+        if (a.serve != nullptr && __builtin_amdgcn_readfirstlane(a.serve->n_active) == 0) break;
+        const bool whole = c0 < big_tiles;
+        const u32 Keff = whole ? K : K / U;
+        const u64 c = c0 + wave;  // this wave's claim number
+        const u64 tile = whole ? c : big_tiles + (c - big_tiles) / U;
+        const u64 sub = whole ? 0 : (c - big_tiles) % U;
+        bool searched = false;
+        if (c < total_claims && (!whole || c < big_tiles))
+            searched = mk_search_claim(a, cx, tile, sub, Keff, c, [] { __builtin_amdgcn_s_barrier(); });
+        if (!searched)
+            for (u32 j = 0; j < Keff * (rows - 2); j++) __builtin_amdgcn_s_barrier();
+    }
+}
+
+// Workgroups of 8 waves (2 per SIMD: one per CU) and of 4 waves (1 per SIMD; with 2 waves
+// per SIMD, two workgroups share a CU and step through the rows independently).
+extern "C" __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
+    mk_search_lock8_kernel(mk_search_args a) {
+    mk_search_lock<8>(a);
+}
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
+    mk_search_lock4_kernel(mk_search_args a) {
+    mk_search_lock<4>(a);
+}
 // Exact check of every candidate: full key with its x sign, matched with the sign
 // known. One thread per candidate slot.
 extern "C" __global__ void __launch_bounds__(64)
@@ -384,12 +512,37 @@ hipError_t mk_k_reset(hipStream_t s, mk_counters* ctr, u32* stop, u32 nstop, con
     return hipGetLastError();
 }
 
-hipError_t mk_k_search(hipStream_t s, u32 grid, const u32* table, u32 w, u32 K, const mk_search_desc* descs,
-                       const mk_seg* segs, u32 nsegs, u64 total_tiles, const u64* masks, const u16* dfas,
-                       u32* spill, mk_counters* ctr, mk_cand* cands, u32 cand_cap, u32* stop,
+hipError_t mk_k_search(hipStream_t s, u32 grid, u32 lock_waves, const u32* table, u32 w, u32 K,
+                       const mk_search_desc* descs, const mk_seg* segs, u32 nsegs, u64 total_tiles, const u64* masks,
+                       const u16* dfas, u32* spill, mk_counters* ctr, mk_cand* cands, u32 cand_cap, u32* stop,
                        const mk_serve_state* serve, u64* next_index) {
-    hipLaunchKernelGGL(mk_search_kernel, dim3(grid), dim3(64), 0, s, table, w, K, descs, segs, nsegs, total_tiles,
-                       masks, dfas, spill, ctr, cands, cand_cap, stop, serve, next_index);
+    mk_search_args a;
+    a.table = table;
+    a.w = w;
+    a.K = K;
+    a.descs = descs;
+    a.segs = segs;
+    a.nsegs = nsegs;
+    a.total_tiles = total_tiles;
+    a.masks = masks;
+    a.dfas = dfas;
+    a.spill = spill;
+    a.ctr = ctr;
+    a.cands = cands;
+    a.cand_cap = cand_cap;
+    a.stop = stop;
+    a.serve = serve;
+    a.next_index = next_index;
+    if (lock_waves == 0) {
+        hipLaunchKernelGGL(mk_search_kernel, dim3(grid), dim3(64), 0, s, a);
+    } else {
+        // grid waves in workgroups of lock_waves (4 or 8, dividing the grid)
+        if ((lock_waves != 4 && lock_waves != 8) || grid % lock_waves != 0) return hipErrorInvalidValue;
+        if (lock_waves == 8)
+            hipLaunchKernelGGL(mk_search_lock8_kernel, dim3(grid / 8), dim3(512), 0, s, a);
+        else
+            hipLaunchKernelGGL(mk_search_lock4_kernel, dim3(grid / 4), dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -252,8 +252,11 @@ MK_DEV mk_niels mk_staged_niels(const uint4* stage, u32 b, u32 lane, i32 d) {
 
 // mk_mul_base_frac with the gathers staged through LDS: the entry of row r + 1 is in
 // flight while row r is added. Every wave of the kernel calls this with all 64 lanes.
+// row_sync() runs once at the top of each of the rows - 2 additions (the search kernel's
+// lockstep variant puts a workgroup barrier there).
+template <class R>
 MK_DEV void mk_mul_base_frac_staged(const u32* __restrict__ table, u32 w, const u32 s[8], uint4* stage, u32 lane,
-                                    fe& yn, fe& yd, fe& xn, fe& xd) {
+                                    fe& yn, fe& yd, fe& xn, fe& xd, R row_sync) {
     mk_digits g = mk_digits_of(s, w);
     const u32 rows = g.rows, cols = mk_cols(w);
     i32 d = mk_next_digit(g);
@@ -264,6 +267,7 @@ MK_DEV void mk_mul_base_frac_staged(const u32* __restrict__ table, u32 w, const 
     mk_ext acc = mk_row0_extended(mk_staged_niels(stage, 0, lane, d));
     MK_NO_UNROLL
     for (u32 row = 1; row + 1 < rows; row++) {
+        row_sync();
         // the copy of this row's entry (issued one row ago) must have landed
         mk_wait_all();
         const mk_niels cur = mk_staged_niels(stage, row & 1, lane, dn);
@@ -274,6 +278,11 @@ MK_DEV void mk_mul_base_frac_staged(const u32* __restrict__ table, u32 w, const 
     }
     mk_wait_all();
     mk_add_niels_frac(acc, mk_staged_niels(stage, (rows - 1) & 1, lane, dn), yn, yd, xn, xd);
+}
+
+MK_DEV void mk_mul_base_frac_staged(const u32* __restrict__ table, u32 w, const u32 s[8], uint4* stage, u32 lane,
+                                    fe& yn, fe& yd, fe& xn, fe& xd) {
+    mk_mul_base_frac_staged(table, w, s, stage, lane, yn, yd, xn, xd, [] {});
 }
 #endif
 
