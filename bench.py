@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--width", type=int, default=24)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--blocks", type=int, default=2)
+    ap.add_argument("--limbs", type=int, default=None,
+                    help="field / table layout: 10 (packed ten-limb), 11 (ten-limb entries), 8 (eight words); "
+                         "default: the library's")
     ap.add_argument("--lockstep", type=int, default=None,
                     help="1: one workgroup per CU adding each comb row together (default: the library's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -74,7 +77,9 @@ def main():
     needle = needle.encode()
     if args.lockstep is None:
         args.lockstep = M.default_options().lockstep
-    gpu = M.Gpu(0, width=args.width, batch=args.batch, blocks=args.blocks, lockstep=args.lockstep)
+    if args.limbs is None:
+        args.limbs = M.default_options().limbs
+    gpu = M.Gpu(0, width=args.width, batch=args.batch, blocks=args.blocks, limbs=args.limbs, lockstep=args.lockstep)
     info = gpu.info()
     print(f"# device {info.name.decode()} CUs={info.compute_units} grid={info.grid_waves} waves "
           f"W={args.width} batch={args.batch} table={info.table_bytes / 2**20:.1f} MiB needle={needle.decode()}",
@@ -119,7 +124,7 @@ def main():
                                "(random.Random(2)), 6-char case-sensitive suffix",
                    "candidates": keys, "needle": needle.decode(), "width": args.width,
                    "keys_per_thread": args.batch, "waves_per_simd": args.blocks, "grid_waves": info.grid_waves,
-                   "lockstep": args.lockstep,
+                   "lockstep": args.lockstep, "limbs": args.limbs,
                    "hits": len(all_hits)},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_MUL / 1e12, "unit": "TOP/s",
                      "frac": achieved / PEAK_MUL, "traffic": traffic,

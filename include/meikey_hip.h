@@ -49,17 +49,22 @@ typedef enum {
 #define MK_MODE_SUFFIX 1
 #define MK_MODE_REGEX 2
 
-/* Field layouts (--gpu-limbs). The reference's codes are "10", "10u", "8", "10k", "10ku"
- * and "5f" (gpu.rs:70-106); this backend implements 10 x 25.5-bit limbs with the table
- * stored as limbs (code "10u"), and accepts "10" as the same layout. */
+/* Field layouts (--gpu-limbs; the reference's Limbs and its codes, gpu.rs:70-106):
+ *   MK_LIMBS_R25  "10"  ten 25.5-bit limbs, table entries packed as 3 x 32 bytes (96 B)
+ *   MK_LIMBS_R25U "10u" ten 25.5-bit limbs, table entries stored as limbs (128 B)
+ *   MK_LIMBS_R32  "8"   eight 32-bit words (full 2^256 range, carries folded by 38), packed entries
+ * The Karatsuba layouts "10k" / "10ku" and the floating-point "5f" are not built
+ * (MK_ERR_OUT_OF_RANGE). */
 #define MK_LIMBS_R25 10
+#define MK_LIMBS_R25U 11
+#define MK_LIMBS_R32 8
 
 /* Kernel settings: --gpu-width / --gpu-batch / --gpu-limbs (src/controller/cli.rs:85-111,
  * gpu.rs:185-240 GpuOptions). */
 typedef struct {
     uint32_t width;  /* comb table width W, 2..28 (reference 2..20); table = rows(W) x (2^(W-1)+1) x 128 B */
     uint32_t batch;  /* keys per thread per tile, 1..256 (one inversion per `batch` keys per lane) */
-    uint32_t limbs;  /* MK_LIMBS_R25 */
+    uint32_t limbs;  /* MK_LIMBS_R25 / MK_LIMBS_R25U / MK_LIMBS_R32 */
     uint32_t blocks; /* resident waves per SIMD the grid is sized for, 1..8 (reference: blocks per SM) */
     uint32_t lockstep; /* 0: one wave per workgroup; 1: the waves of a CU form one workgroup and add each
                           comb row together, which keeps the CU's address translation on one or two rows
@@ -146,10 +151,12 @@ mk_status mk_derive_noalloc(mk_gpu* g, const uint8_t key[32], int seed_source, u
 /* Public keys of explicit seeds (n x 32 bytes in, n x 32 bytes out). */
 mk_status mk_derive_seeds(mk_gpu* g, const uint8_t* seeds, uint32_t n, uint8_t* pks);
 
-/* Raw comb-table entries (row, k) as stored on the device: 32 words each, three 10-limb
- * field elements, the reference's (y+x, y-x, 2dxy) halved: (y+x)/2, (y-x)/2, dxy (row 0:
- * xy). Limbs 0-7 of each in words 0-7 / 8-15 / 16-23, limbs 8-9 in words 24-25 / 26-27 /
- * 28-29, words 30-31 zero. For tests of the table build. */
+/* Raw comb-table entries (row, k) as stored on the device, for tests of the table build.
+ * Each holds the reference's (y+x, y-x, 2dxy) halved: (y+x)/2, (y-x)/2, dxy (row 0: xy), every
+ * field canonical. MK_LIMBS_R25U: 32 words, three 10-limb fields, limbs 0-7 of each in words
+ * 0-7 / 8-15 / 16-23, limbs 8-9 in words 24-25 / 26-27 / 28-29, words 30-31 zero. Packed
+ * layouts (MK_LIMBS_R25, MK_LIMBS_R32): 24 words, field f as a little-endian 256-bit value in
+ * words 8f .. 8f+7. */
 mk_status mk_table_read(mk_gpu* g, uint32_t row, uint32_t k, uint32_t count, uint32_t* words);
 
 /* ------------------------------------------------------------------------------------
@@ -315,8 +322,9 @@ typedef struct {
 } mk_tune_result;
 
 mk_status mk_fastest(int ordinal, const mk_options* candidates, uint32_t n, mk_tune_result* results, mk_options* best);
-/* Stage 1: widths 16/20/22/24; stage 2: 32/64/128/256 keys per lane; stage 3: 1 or 2 waves
- * per SIMD. results (cap entries) receive every measured setting; *n the count. */
+/* Stage 1: the layouts "10u", "10" and "8" at W = 16 (gpu.rs:364-370); stage 2: widths
+ * 16/20/22/24; stage 3: 32/64/128/256 keys per lane; stage 4: 1 or 2 waves per SIMD, and
+ * lockstep workgroups. results (cap entries) receive every measured setting; *n the count. */
 mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* results, uint32_t cap, uint32_t* n);
 
 /* OpenSSH text of a key: the authorized_keys line "ssh-ed25519 <b64>[ <comment>]"

@@ -20,7 +20,10 @@ SEED_COUNTER = 1
 MODE_CONTAINS = 0
 MODE_SUFFIX = 1
 MODE_REGEX = 2
-LIMBS_R25 = 10
+LIMBS_R25 = 10   # "10": ten 25.5-bit limbs, packed 96-byte table entries
+LIMBS_R25U = 11  # "10u": ten 25.5-bit limbs, 128-byte entries of limbs
+LIMBS_R32 = 8    # "8": eight 32-bit words, packed entries
+ENTRY_WORDS = {LIMBS_R25: 24, LIMBS_R25U: 32, LIMBS_R32: 24}
 
 STATUS = {
     0: "ok",
@@ -227,8 +230,14 @@ def openssh_private(seed: bytes, pk: bytes, comment: str = "", checkint: Optiona
 
 
 def fastest(ordinal: int, candidates):
-    """Measure each (width, batch, blocks) candidate: (best Options, rows as fastest_staged)."""
-    cands = (Options * len(candidates))(*[Options(w, b, LIMBS_R25, k) for w, b, k in candidates])
+    """Measure each (width, batch, blocks[, limbs[, lockstep]]) candidate: (best Options,
+    rows as fastest_staged). Limbs default to the library's, lockstep to 0."""
+    lim = default_options().limbs
+    opts = []
+    for c in candidates:
+        w, b, k = c[:3]
+        opts.append(Options(w, b, c[3] if len(c) > 3 else lim, k, c[4] if len(c) > 4 else 0))
+    cands = (Options * len(opts))(*opts)
     res = (TuneResult * len(candidates))()
     best = Options()
     _check(lib().mk_fastest(ordinal, cands, len(candidates), res, ctypes.byref(best)))
@@ -242,7 +251,8 @@ def fastest_staged(ordinal: int = 0):
     res = (TuneResult * 64)()
     n = ctypes.c_uint32(0)
     _check(lib().mk_fastest_staged(ordinal, ctypes.byref(best), res, 64, ctypes.byref(n)))
-    rows = [(r.options.width, r.options.batch, r.options.blocks, r.keys_per_s, r.status) for r in res[:n.value]]
+    rows = [(r.options.width, r.options.batch, r.options.blocks, r.keys_per_s, r.status, r.options.limbs,
+             r.options.lockstep) for r in res[:n.value]]
     return best, rows
 
 
@@ -388,8 +398,10 @@ class Launch:
 class Gpu:
     """One device with its comb table (the reference's Gpu + Kernel)."""
 
-    def __init__(self, ordinal: int = 0, width: int = 24, batch: int = 256, blocks: int = 2, limbs: int = LIMBS_R25,
-                 lockstep: int = 0):
+    def __init__(self, ordinal: int = 0, width: int = 24, batch: int = 256, blocks: int = 2,
+                 limbs: Optional[int] = None, lockstep: int = 0):
+        if limbs is None:
+            limbs = default_options().limbs
         self.options = Options(width, batch, limbs, blocks, lockstep)
         h = ctypes.c_void_p()
         _check(lib().mk_open(ordinal, ctypes.byref(self.options), ctypes.byref(h)))
@@ -472,13 +484,14 @@ class Gpu:
         return a.value, b.value
 
     def table_words(self, row: int, k: int, count: int = 1, as_array: bool = False):
-        """Raw table entries from (row, k) on, 32 words each (a list, or with as_array a
-        numpy array of shape (count, 32))."""
-        buf = (ctypes.c_uint32 * (32 * count))()
+        """Raw table entries from (row, k) on, 32 words each for LIMBS_R25U, 24 for the packed
+        layouts (a list, or with as_array a numpy array of shape (count, words))."""
+        n = ENTRY_WORDS[self.options.limbs]
+        buf = (ctypes.c_uint32 * (n * count))()
         _check(lib().mk_table_read(self._h, row, k, count, buf))
         if as_array:
             import numpy as np
-            return np.frombuffer(buf, dtype=np.uint32).reshape(count, 32).copy()
+            return np.frombuffer(buf, dtype=np.uint32).reshape(count, n).copy()
         return list(buf)
 
     def batch_run(self, specs: Sequence[dict], cap: int = 1 << 20, as_array: bool = False):

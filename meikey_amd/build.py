@@ -27,7 +27,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-mllvm", "-amdgpu-sched-strategy=max-ilp"]
-HIP_SOURCES = [os.path.join(CSRC, "mk_kernels.hip"), os.path.join(CSRC, "mk_host.cpp"), os.path.join(CSRC, "mk_pattern.cpp"), os.path.join(CSRC, "mk_serve.cpp"), os.path.join(CSRC, "mk_driver.cpp")]
+# mk_kernels.hip builds the "10u" layout; mk_kernels_p10.hip and mk_kernels_r32.hip include
+# it for the packed ten-limb and the eight-word layouts.
+HIP_SOURCES = [os.path.join(CSRC, f) for f in ("mk_kernels.hip", "mk_kernels_p10.hip", "mk_kernels_r32.hip",
+                                                 "mk_host.cpp", "mk_pattern.cpp", "mk_serve.cpp", "mk_driver.cpp")]
 
 
 def _run(cmd: list[str]) -> None:
@@ -59,11 +62,21 @@ def build_hip(out: str = LIB) -> str:
     return out
 
 
+# The CPU test builds of the arithmetic headers, one per layout (as the kernels are built).
+HOSTCHECK_LAYOUTS = {"": [], "_p10": ["-DMK_PACKED=1"], "_r32": ["-DMK_FIELD=8", "-DMK_PACKED=1"]}
+
+
+def hostcheck_path(layout: str = "") -> str:
+    return os.path.join(HERE, f"libmeikey_hostcheck{layout}.so")
+
+
 def build_hostcheck(out: str = HOSTCHECK) -> str:
-    tmp = out + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-function", "-o", tmp,
-          os.path.join(CSRC, "mk_hostcheck.cpp")])
-    os.replace(tmp, out)
+    for layout, defs in HOSTCHECK_LAYOUTS.items():
+        path = out if layout == "" else hostcheck_path(layout)
+        tmp = path + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-function", *defs, "-o", tmp,
+              os.path.join(CSRC, "mk_hostcheck.cpp")])
+        os.replace(tmp, path)
     return out
 
 

@@ -48,9 +48,23 @@ typedef uint16_t u16;
 MK_DEV u32 mk_rows(u32 w) { return (MK_M_BITS + w - 1) / w; }
 MK_DEV u32 mk_cols(u32 w) { return (1u << (w - 1)) + 1; }
 
-// One comb-table entry: the affine point as ((y+x)/2, (y-x)/2, dxy) (row 0: xy), three
-// 10-limb field elements (30 words) plus 2 pad words, so an entry is one 128-byte cache line.
-#define MK_ENTRY_WORDS 32
+// Field representation and comb-table layout of one build of the kernels (the reference's
+// --gpu-limbs codes, src/service/gpu.rs:70-106). Each is compiled into its own set of
+// kernels; mk_options.limbs picks the set at run time.
+//   MK_FIELD 10: ten 25.5-bit limbs (mk_field.h); 8: eight 32-bit words (mk_field8.h)
+//   MK_PACKED 0: a table entry holds its three fields as limbs: 30 words + 2 pad words,
+//                one 128-byte line ("10u");
+//             1: a table entry holds them as canonical little-endian 256-bit values,
+//                3 x 8 words = 96 bytes ("10", "8")
+#ifndef MK_FIELD
+#define MK_FIELD 10
+#endif
+#ifndef MK_PACKED
+#define MK_PACKED 0
+#endif
+#define MK_FE_LIMBS MK_FIELD
+// One comb-table entry: the affine point as ((y+x)/2, (y-x)/2, dxy) (row 0: xy).
+#define MK_ENTRY_WORDS (MK_PACKED ? 24 : 32)
 
 // The 44 base64 characters that depend on the key (blob bytes 18..50), and the
 // first of them that a pattern may match (reference src/service/gpu/kernel.cu:112-119).

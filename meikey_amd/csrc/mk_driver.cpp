@@ -274,16 +274,28 @@ mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* resul
     };
     mk_options base;
     mk_default_options(&base);
-    // stage 1: comb widths at the default keys per lane and waves per SIMD
+    // stage 1: the field / table layouts (gpu.rs:364-370 measures every Limbs first) at
+    // W = 16 and the default keys per lane and waves per SIMD
+    std::vector<mk_options> c0;
+    for (uint32_t limbs : {(uint32_t)MK_LIMBS_R25U, (uint32_t)MK_LIMBS_R25, (uint32_t)MK_LIMBS_R32}) {
+        mk_options o = base;
+        o.width = 16;
+        o.limbs = limbs;
+        c0.push_back(o);
+    }
+    mk_options w0 = base;
+    mk_status st = stage(c0, w0);
+    // stage 2: comb widths with that layout
     std::vector<mk_options> c1;
     for (uint32_t w : {16u, 20u, 22u, 24u}) {
         mk_options o = base;
+        o.limbs = w0.limbs;
         o.width = w;
         c1.push_back(o);
     }
-    mk_options w1 = base;
-    mk_status st = stage(c1, w1);
-    // stage 2: keys per lane (inversion batch) at the best width
+    mk_options w1 = w0;
+    if (st == MK_OK) st = stage(c1, w1);
+    // stage 3: keys per lane (inversion batch) at the best width
     std::vector<mk_options> c2;
     for (uint32_t b : {32u, 64u, 128u, 256u}) {
         if (b == w1.batch) continue;
@@ -296,12 +308,13 @@ mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* resul
         c2.push_back(w1);
         st = stage(c2, w2);
     }
-    // stage 3: resident waves per SIMD
+    // stage 4: resident waves per SIMD, and workgroups that add each comb row in lockstep
     mk_options w3 = w2;
     if (st == MK_OK) {
-        mk_options o = w2;
+        mk_options o = w2, l = w2;
         o.blocks = w2.blocks == 2 ? 1 : 2;
-        st = stage({w2, o}, w3);
+        l.lockstep = 2;
+        st = stage({w2, o, l}, w3);
     }
     if (n_out) *n_out = (uint32_t)all.size();
     if (results)

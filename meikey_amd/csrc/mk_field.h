@@ -20,6 +20,10 @@
 
 #include "mk_common.h"
 
+#if MK_FIELD == 8
+#include "mk_field8.h"
+#else
+
 struct fe {
     u32 v[10];
 };
@@ -219,3 +223,5 @@ MK_DEV bool fe_is_odd(const fe& a) {
     fe_to_words(a, w);
     return (w[0] & 1u) != 0;
 }
+
+#endif  // MK_FIELD

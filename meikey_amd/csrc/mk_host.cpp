@@ -83,11 +83,11 @@ mk_status mk_run_step(mk_gpu* g, const mk_search_desc* descs, const u64* masks, 
     MK_HIP(mk_k_reset(g->stream, g->ctr, stop, nstop, nullptr));
     MK_HIP(hipEventRecord(g->ev[0], g->stream));
     if (total_tiles > 0)
-        MK_HIP(mk_k_search(g->stream, g->grid, g->lock_waves, g->table, g->opt.width, g->opt.batch, descs, segs, nsegs, total_tiles,
+        MK_HIP(g->ks->search(g->stream, g->grid, g->lock_waves, g->table, g->opt.width, g->opt.batch, descs, segs, nsegs, total_tiles,
                            masks, dfas, g->spill, g->ctr, g->cands, kCandCap, stop, nullptr, nullptr));
     MK_HIP(hipEventRecord(g->ev[1], g->stream));
     if (total_tiles > 0)
-        MK_HIP(mk_k_confirm(g->stream, g->table, g->opt.width, descs, masks, dfas, g->ctr, g->cands, kCandCap, g->hits,
+        MK_HIP(g->ks->confirm(g->stream, g->table, g->opt.width, descs, masks, dfas, g->ctr, g->cands, kCandCap, g->hits,
                             kHitCap));
     MK_HIP(hipEventRecord(g->ev[2], g->stream));
     MK_HIP(hipMemcpyAsync(g->h_ctr, g->ctr, sizeof(mk_counters), hipMemcpyDeviceToHost, g->stream));
@@ -229,7 +229,7 @@ void mk_default_options(mk_options* out) {
     if (!out) return;
     out->width = 24;
     out->batch = 256;
-    out->limbs = MK_LIMBS_R25;
+    out->limbs = MK_LIMBS_R25U;
     out->blocks = 2;
     out->lockstep = 0;
 }
@@ -246,7 +246,12 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
                     "the table width must be %d-%d, the keys per thread 1-%u and the waves per SIMD 1-8 (W=%u, "
                     "batch %u, blocks %u)",
                     MK_MIN_W, MK_MAX_W, kMaxBatch, opt.width, opt.batch, opt.blocks);
-    if (opt.limbs != MK_LIMBS_R25) return fail(MK_ERR_OUT_OF_RANGE, "limb layout %u is not available", opt.limbs);
+    const mk_kernel_set* ks = opt.limbs == MK_LIMBS_R25U  ? &mk_kernels_u10
+                              : opt.limbs == MK_LIMBS_R25 ? &mk_kernels_p10
+                              : opt.limbs == MK_LIMBS_R32 ? &mk_kernels_r32
+                                                          : nullptr;
+    if (!ks)
+        return fail(MK_ERR_OUT_OF_RANGE, "limb layout %u is not available (10, 11 = \"10u\" or 8)", opt.limbs);
     if (opt.lockstep > 2 || (opt.lockstep == 1 && opt.blocks > 2))
         return fail(MK_ERR_OUT_OF_RANGE,
                     "lockstep is 0, 1 (one workgroup per CU, 1 or 2 waves per SIMD) or 2 (workgroups of one wave per "
@@ -260,6 +265,7 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
     mk_gpu* g = new mk_gpu();
     g->device = ordinal;
     g->opt = opt;
+    g->ks = ks;
     void* bases = nullptr;  // row bases, needed only while the table is built
     auto bail = [&](mk_status s) {
         if (bases) (void)hipFree(bases);
@@ -289,14 +295,14 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
     for (auto& e : g->ev) MK_HIP_G(hipEventCreate(&e));
 
     // comb table, built on the device
-    g->table_bytes = (u64)g->rows * g->cols * MK_ENTRY_WORDS * 4;
+    g->table_bytes = (u64)g->rows * g->cols * g->ks->entry_words * 4;
     MK_HIP_G(hipMalloc((void**)&g->table, g->table_bytes));
     MK_HIP_G(hipMalloc(&bases, (size_t)(g->rows + 1) * MK_EXT_BYTES));
-    MK_HIP_G(mk_k_table_bases(g->stream, opt.width, bases));
+    MK_HIP_G(g->ks->table_bases(g->stream, opt.width, bases));
     const u64 entries = (u64)g->rows * g->cols;
     const u64 chunk = 1ull << 22;
     for (u64 e0 = 0; e0 < entries; e0 += chunk)
-        MK_HIP_G(mk_k_table_fill(g->stream, opt.width, bases, e0, std::min(chunk, entries - e0), g->table));
+        MK_HIP_G(g->ks->table_fill(g->stream, opt.width, bases, e0, std::min(chunk, entries - e0), g->table));
     MK_HIP_G(hipStreamSynchronize(g->stream));
     MK_HIP_G(hipFree(bases));
     bases = nullptr;
@@ -459,7 +465,7 @@ mk_status mk_derive(mk_gpu* g, const uint8_t key[32], int seed_source, uint64_t 
     if (e == hipSuccess) e = hipMemcpy(dkey, kw, 32, hipMemcpyHostToDevice);
     for (u64 off = 0; e == hipSuccess && off < n; off += chunk) {
         const u64 m = std::min(chunk, n - off);
-        e = mk_k_derive(g->stream, g->table, g->opt.width, dkey, (u32)seed_source, first + off, m, dpk, dseed);
+        e = g->ks->derive(g->stream, g->table, g->opt.width, dkey, (u32)seed_source, first + off, m, dpk, dseed);
         if (e == hipSuccess) e = hipMemcpyAsync(pks + 32 * off, dpk, m * 32, hipMemcpyDeviceToHost, g->stream);
         if (e == hipSuccess && seeds)
             e = hipMemcpyAsync(seeds + 32 * off, dseed, m * 32, hipMemcpyDeviceToHost, g->stream);
@@ -492,7 +498,7 @@ mk_status mk_derive_noalloc(mk_gpu* g, const uint8_t key[32], int seed_source, u
     // back is written by this call's kernel, so nothing is cleared
     u32* dpk = g->noalloc;
     u32* dseed = seeds ? g->noalloc + 8 * MK_NOALLOC_KEYS : nullptr;
-    MK_HIP(mk_k_derive_kv(g->stream, g->table, g->opt.width, kw, (u32)seed_source, first, n, dpk, dseed));
+    MK_HIP(g->ks->derive_kv(g->stream, g->table, g->opt.width, kw, (u32)seed_source, first, n, dpk, dseed));
     MK_HIP(hipMemcpyAsync(pks, dpk, (size_t)n * 32, hipMemcpyDeviceToHost, g->stream));
     if (seeds) MK_HIP(hipMemcpyAsync(seeds, dseed, (size_t)n * 32, hipMemcpyDeviceToHost, g->stream));
     MK_HIP(hipStreamSynchronize(g->stream));
@@ -507,7 +513,7 @@ mk_status mk_derive_seeds(mk_gpu* g, const uint8_t* seeds, uint32_t n, uint8_t* 
     MK_HIP(hipMalloc((void**)&ds, (size_t)n * 32));
     hipError_t e = hipMalloc((void**)&dp, (size_t)n * 32);
     if (e == hipSuccess) e = hipMemcpy(ds, seeds, (size_t)n * 32, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = mk_k_derive_seeds(g->stream, g->table, g->opt.width, ds, n, dp);
+    if (e == hipSuccess) e = g->ks->derive_seeds(g->stream, g->table, g->opt.width, ds, n, dp);
     if (e == hipSuccess) e = hipMemcpyAsync(pks, dp, (size_t)n * 32, hipMemcpyDeviceToHost, g->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
     (void)hipFree(ds);
@@ -520,8 +526,8 @@ mk_status mk_table_read(mk_gpu* g, uint32_t row, uint32_t k, uint32_t count, uin
     if (!g || (count && !words)) return fail(MK_ERR_INVALID, "null pointer");
     if (row >= g->rows || (u64)k + count > g->cols) return fail(MK_ERR_OUT_OF_RANGE, "entry (%u, %u+%u)", row, k, count);
     MK_HIP(hipSetDevice(g->device));
-    const u64 off = ((u64)row * g->cols + k) * MK_ENTRY_WORDS;
-    MK_HIP(hipMemcpy(words, g->table + off, (size_t)count * MK_ENTRY_WORDS * 4, hipMemcpyDeviceToHost));
+    const u64 off = ((u64)row * g->cols + k) * g->ks->entry_words;
+    MK_HIP(hipMemcpy(words, g->table + off, (size_t)count * g->ks->entry_words * 4, hipMemcpyDeviceToHost));
     return MK_OK;
 }
 

@@ -39,36 +39,50 @@ void hc_secret_scalar_counter(const u32 key[8], u64 index, u32 s[8]) {
     mk_secret_scalar_mid(mid, mk_be_word((u32)t, (u32)(t >> 32)), s);
 }
 
-void hc_fe_mul_limbs(const u32 f[10], const u32 g[10], u32 out[10]) {
+// The build's field and table layout: 10 * MK_FIELD + MK_PACKED (100 "10u", 101 "10", 81 "8").
+int hc_layout(void) { return 10 * MK_FIELD + MK_PACKED; }
+
+#define FE_BYTES (4 * MK_FE_LIMBS)
+
+void hc_fe_mul_limbs(const u32* f, const u32* g, u32* out) {
     fe a, b;
-    memcpy(a.v, f, 40);
-    memcpy(b.v, g, 40);
+    memcpy(a.v, f, FE_BYTES);
+    memcpy(b.v, g, FE_BYTES);
     const fe r = fe_mul(a, b);
-    memcpy(out, r.v, 40);
+    memcpy(out, r.v, FE_BYTES);
 }
 
-void hc_fe_sq_limbs(const u32 f[10], u32 out[10]) {
+void hc_fe_sq_limbs(const u32* f, u32* out) {
     fe a;
-    memcpy(a.v, f, 40);
+    memcpy(a.v, f, FE_BYTES);
     const fe r = fe_sq(a);
-    memcpy(out, r.v, 40);
+    memcpy(out, r.v, FE_BYTES);
 }
 
-void hc_fe_to_words(const u32 f[10], u32 out[8]) {
+// op on raw limbs (no reduction of the inputs): 0 add, 1 sub, 2 mul, 5 neg; out raw limbs.
+void hc_fe_op_limbs(u32 op, const u32* f, const u32* g, u32* out) {
+    fe a, b;
+    memcpy(a.v, f, FE_BYTES);
+    memcpy(b.v, g, FE_BYTES);
+    fe r = op == 0 ? fe_add(a, b) : op == 1 ? fe_sub(a, b) : op == 2 ? fe_mul(a, b) : fe_neg(a);
+    memcpy(out, r.v, FE_BYTES);
+}
+
+void hc_fe_to_words(const u32* f, u32 out[8]) {
     fe a;
-    memcpy(a.v, f, 40);
+    memcpy(a.v, f, FE_BYTES);
     fe_to_words(a, out);
 }
 
-void hc_fe_carried_to_words(const u32 f[10], u32 out[8]) {
+void hc_fe_carried_to_words(const u32* f, u32 out[8]) {
     fe a;
-    memcpy(a.v, f, 40);
+    memcpy(a.v, f, FE_BYTES);
     fe_carried_to_words(a, out);
 }
 
-void hc_fe_from_words(const u32 w[8], u32 out[10]) {
+void hc_fe_from_words(const u32 w[8], u32* out) {
     const fe a = fe_from_words(w);
-    memcpy(out, a.v, 40);
+    memcpy(out, a.v, FE_BYTES);
 }
 
 // op: 0 add, 1 sub, 2 mul, 3 sq, 4 invert, 5 neg; a, b 256-bit words; out canonical.
@@ -122,7 +136,7 @@ int hc_lane(const u32* table, u32 w, const u32 key[8], u32 source, u64 first, u3
     mt.len = m.len;
     mt.mask = m.mask;
     mt.dfa = m.dfa.empty() ? nullptr : m.dfa.data();
-    std::vector<u32> spill((size_t)K * 20);
+    std::vector<u32> spill((size_t)K * 2 * MK_FE_LIMBS);
     mk_spill sp;
     sp.base = spill.data();
     sp.stride = 1;

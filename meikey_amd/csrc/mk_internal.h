@@ -47,6 +47,7 @@ struct mk_gpu {
     int device = 0;
     hipStream_t stream = nullptr;
     mk_options opt{};
+    const mk_kernel_set* ks = nullptr;  // the kernels of opt.limbs
     u32 cus = 0, grid = 0, rows = 0, cols = 0;
     u32 lock_waves = 0;  // waves per workgroup of the lockstep search kernel (0: one-wave workgroups)
     u64 lanes = 0, hbm = 0;

@@ -18,11 +18,31 @@
 // with a candidate append it through one wave ballot and one atomic.
 // The confirm kernel re-derives each candidate with x, matches exactly and writes
 // exact hits (index, seed, public key).
+//
+// This file is compiled once per field representation and table layout (mk_common.h):
+// as it stands for ten 25.5-bit limbs and 128-byte entries of limbs ("10u"), and through
+// mk_kernels_p10.hip (packed 96-byte entries, "10") and mk_kernels_r32.hip (eight 32-bit
+// words, "8"), which set MK_FIELD / MK_PACKED / MK_VARIANT first. Each build lives in its
+// own namespace with kernel names suffixed by MK_VARIANT, and exports one mk_kernel_set.
 #include <hip/hip_runtime.h>
+
+#include <mutex>
+
+#include "mk_launch.h"
+
+#ifndef MK_VARIANT
+#define MK_VARIANT _u10
+#define MK_BASE_TU 1
+#endif
+#define MK_CAT2(a, b) a##b
+#define MK_CAT(a, b) MK_CAT2(a, b)
+#define MK_K(name) MK_CAT(name, MK_VARIANT)
+#define MK_NS MK_K(mk_ns)
+
+namespace MK_NS {
 
 #include "mk_field.h"
 #include "mk_hash.h"
-#include "mk_kernels.h"
 #include "mk_match.h"
 #include "mk_point.h"
 #include "mk_tile.h"
@@ -30,13 +50,13 @@
 // -------------------------------------------------------------- table build
 
 // Row bases and the top row's offset (mk_row_bases). One thread.
-extern "C" __global__ void mk_table_bases_kernel(u32 w, mk_ext* out) {
+extern "C" __global__ void MK_K(mk_table_bases_kernel)(u32 w, mk_ext* out) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     mk_row_bases(w, out);
 }
 
 // One thread per entry, entries [e0, e0 + n) of the row-major table.
-extern "C" __global__ void __launch_bounds__(64) mk_table_fill_kernel(u32 w, const mk_ext* __restrict__ bases,
+extern "C" __global__ void __launch_bounds__(64) MK_K(mk_table_fill_kernel)(u32 w, const mk_ext* __restrict__ bases,
                                                                       u64 e0, u64 n, u32* __restrict__ table) {
     const u64 e = e0 + (u64)blockIdx.x * 64 + threadIdx.x;
     if (e >= e0 + n) return;
@@ -62,7 +82,7 @@ MK_DEV void mk_derive_one(const u32* __restrict__ table, u32 w, const u32 k[8], 
 
 // Public keys (and optionally seeds) of indices first .. first + n - 1 of one search
 // key. One thread per key.
-extern "C" __global__ void __launch_bounds__(64) mk_derive_kernel(const u32* __restrict__ table, u32 w,
+extern "C" __global__ void __launch_bounds__(64) MK_K(mk_derive_kernel)(const u32* __restrict__ table, u32 w,
                                                                   const u32* __restrict__ key, u32 source,
                                                                   u64 first, u64 n, u32* __restrict__ out_pk,
                                                                   u32* __restrict__ out_seed) {
@@ -74,7 +94,7 @@ extern "C" __global__ void __launch_bounds__(64) mk_derive_kernel(const u32* __r
 }
 
 // The same with the key passed by value, so the launch needs no device copy of it.
-extern "C" __global__ void __launch_bounds__(64) mk_derive_kv_kernel(const u32* __restrict__ table, u32 w,
+extern "C" __global__ void __launch_bounds__(64) MK_K(mk_derive_kv_kernel)(const u32* __restrict__ table, u32 w,
                                                                      mk_key_words key, u32 source, u64 first,
                                                                      u64 n, u32* __restrict__ out_pk,
                                                                      u32* __restrict__ out_seed) {
@@ -83,13 +103,8 @@ extern "C" __global__ void __launch_bounds__(64) mk_derive_kv_kernel(const u32* 
     mk_derive_one(table, w, key.w, source, first, i, out_pk, out_seed);
 }
 
-// Staging of the allocation-free derivation (mk_derive_noalloc): MK_NOALLOC_KEYS public
-// keys, then as many seeds. File scope, one copy per device for the life of the process;
-// host code reaches it only through the address hipGetSymbolAddress gives per device.
-static __device__ __align__(256) u32 scratch[MK_NOALLOC_WORDS];
-
 // Public keys of explicit 32-byte seeds (known-answer self-check).
-extern "C" __global__ void __launch_bounds__(64) mk_derive_seeds_kernel(const u32* __restrict__ table, u32 w,
+extern "C" __global__ void __launch_bounds__(64) MK_K(mk_derive_seeds_kernel)(const u32* __restrict__ table, u32 w,
                                                                         const u32* __restrict__ seeds, u32 n,
                                                                         u32* __restrict__ out_pk) {
     const u32 i = blockIdx.x * 64 + threadIdx.x;
@@ -115,8 +130,9 @@ __device__ static u64 mk_uniform64(u64 v) {
 #define MK_SEARCH_WAVES 2
 #endif
 
-// Largest DFA that fits the 16 KB staging buffers (states x 65 x 2 bytes).
-#define MK_LDS_DFA_STATES 126
+// Largest DFA that fits a wave's staging buffers (2 x MK_STAGE_CHUNKS x 1 KB; states x 65
+// x 2 bytes): 126 states with 128-byte entries, 94 with 96-byte ones.
+#define MK_LDS_DFA_STATES ((2 * MK_STAGE_CHUNKS * 64 * 16) / 130)
 
 // Orders this wave's LDS writes before its later LDS reads by other lanes (the per-wave
 // buffers are shared by no other wave, so no workgroup barrier is needed).
@@ -295,9 +311,9 @@ __device__ __forceinline__ bool mk_search_claim(const mk_search_args& a, mk_wave
 
 // The search kernel: one wave per workgroup, each wave claims its own tiles.
 extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
-    mk_search_kernel(mk_search_args a) {
+    MK_K(mk_search_kernel)(mk_search_args a) {
     __shared__ u64 lmask[64];
-    __shared__ uint4 stage[2 * 8 * 64];
+    __shared__ uint4 stage[2 * MK_STAGE_CHUNKS * 64];
     __shared__ mk_sha_mid smid;
     mk_wave_ctx cx;
     cx.lane = threadIdx.x;
@@ -342,7 +358,7 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
 template <u32 NW>
 __device__ __forceinline__ void mk_search_lock(const mk_search_args& a) {
     __shared__ u64 lmask[NW][64];
-    __shared__ uint4 stage[NW][2 * 8 * 64];
+    __shared__ uint4 stage[NW][2 * MK_STAGE_CHUNKS * 64];
     __shared__ mk_sha_mid smid[NW];
     __shared__ u64 wg_claim[2];
     const u32 nw = NW;  // waves of this workgroup
@@ -390,17 +406,17 @@ __device__ __forceinline__ void mk_search_lock(const mk_search_args& a) {
 // Workgroups of 8 waves (2 per SIMD: one per CU) and of 4 waves (1 per SIMD; with 2 waves
 // per SIMD, two workgroups share a CU and step through the rows independently).
 extern "C" __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
-    mk_search_lock8_kernel(mk_search_args a) {
+    MK_K(mk_search_lock8_kernel)(mk_search_args a) {
     mk_search_lock<8>(a);
 }
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
-    mk_search_lock4_kernel(mk_search_args a) {
+    MK_K(mk_search_lock4_kernel)(mk_search_args a) {
     mk_search_lock<4>(a);
 }
 // Exact check of every candidate: full key with its x sign, matched with the sign
 // known. One thread per candidate slot.
 extern "C" __global__ void __launch_bounds__(64)
-    mk_confirm_kernel(const u32* __restrict__ table, u32 w, const mk_search_desc* __restrict__ descs,
+    MK_K(mk_confirm_kernel)(const u32* __restrict__ table, u32 w, const mk_search_desc* __restrict__ descs,
                       const u64* __restrict__ masks, const u16* __restrict__ dfas, mk_counters* __restrict__ ctr,
                       const mk_cand* __restrict__ cands, u32 cand_cap, mk_hit* __restrict__ hits, u32 hit_cap) {
     const u32 i = blockIdx.x * 64 + threadIdx.x;
@@ -432,87 +448,41 @@ extern "C" __global__ void __launch_bounds__(64)
     for (int j = 0; j < 8; j++) h.pk[j] = pk[j];
 }
 
-// Clears the launch counters and the first nstop stop flags with a kernel (a plain
-// fill rather than a memset node, so the step can be captured in a graph).
-// With `serve`, the stop flags of the step's active slots are cleared instead.
-extern "C" __global__ void mk_reset_kernel(mk_counters* ctr, u32* stop, u32 nstop, const mk_serve_state* serve) {
-    const u32 i = blockIdx.x * 64 + threadIdx.x;
-    if (i == 0) {
-        ctr->queue = 0;
-        ctr->queue2 = 0;
-        ctr->n_cands = 0;
-        ctr->n_hits = 0;
-        ctr->overflow = 0;
-    }
-    if (serve != nullptr) {
-        if (i < serve->n_active && i < MK_SERVE_MAX_SLOTS) stop[serve->slots[i]] = 0;
-    } else if (stop != nullptr && i < nstop) {
-        stop[i] = 0;
-    }
-}
-
 // ------------------------------------------------------------ host launchers
 
-#include <mutex>
-
-#include "mk_launch.h"
-
-hipError_t mk_k_table_bases(hipStream_t s, u32 w, void* bases) {
-    hipLaunchKernelGGL(mk_table_bases_kernel, dim3(1), dim3(64), 0, s, w, (mk_ext*)bases);
+static hipError_t mk_k_table_bases(hipStream_t s, u32 w, void* bases) {
+    hipLaunchKernelGGL(MK_K(mk_table_bases_kernel), dim3(1), dim3(64), 0, s, w, (mk_ext*)bases);
     return hipGetLastError();
 }
 
-hipError_t mk_k_table_fill(hipStream_t s, u32 w, const void* bases, u64 e0, u64 n, u32* table) {
+static hipError_t mk_k_table_fill(hipStream_t s, u32 w, const void* bases, u64 e0, u64 n, u32* table) {
     const u64 blocks = (n + 63) / 64;
-    hipLaunchKernelGGL(mk_table_fill_kernel, dim3((u32)blocks), dim3(64), 0, s, w, (const mk_ext*)bases, e0, n, table);
+    hipLaunchKernelGGL(MK_K(mk_table_fill_kernel), dim3((u32)blocks), dim3(64), 0, s, w, (const mk_ext*)bases, e0, n, table);
     return hipGetLastError();
 }
 
-hipError_t mk_k_derive(hipStream_t s, const u32* table, u32 w, const u32* key, u32 source, u64 first, u64 n,
+static hipError_t mk_k_derive(hipStream_t s, const u32* table, u32 w, const u32* key, u32 source, u64 first, u64 n,
                        u32* out_pk, u32* out_seed) {
     const u64 blocks = (n + 63) / 64;
-    hipLaunchKernelGGL(mk_derive_kernel, dim3((u32)blocks), dim3(64), 0, s, table, w, key, source, first, n, out_pk,
+    hipLaunchKernelGGL(MK_K(mk_derive_kernel), dim3((u32)blocks), dim3(64), 0, s, table, w, key, source, first, n, out_pk,
                        out_seed);
     return hipGetLastError();
 }
 
-hipError_t mk_k_derive_kv(hipStream_t s, const u32* table, u32 w, const mk_key_words& key, u32 source, u64 first,
+static hipError_t mk_k_derive_kv(hipStream_t s, const u32* table, u32 w, const mk_key_words& key, u32 source, u64 first,
                           u64 n, u32* out_pk, u32* out_seed) {
     const u64 blocks = (n + 63) / 64;
-    hipLaunchKernelGGL(mk_derive_kv_kernel, dim3((u32)blocks), dim3(64), 0, s, table, w, key, source, first, n,
+    hipLaunchKernelGGL(MK_K(mk_derive_kv_kernel), dim3((u32)blocks), dim3(64), 0, s, table, w, key, source, first, n,
                        out_pk, out_seed);
     return hipGetLastError();
 }
 
-// Device address of `scratch` on the current device, looked up once per device (each
-// device holds its own copy at its own address) and kept for the life of the process.
-hipError_t mk_k_noalloc_scratch(int device, u32** out) {
-    static std::mutex mu;
-    static u32* ptrs[MK_MAX_DEVICES] = {};
-    if (device < 0 || device >= MK_MAX_DEVICES) return hipErrorInvalidDevice;
-    std::lock_guard<std::mutex> lock(mu);
-    if (ptrs[device] == nullptr) {
-        void* p = nullptr;
-        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
-        if (e != hipSuccess) return e;
-        ptrs[device] = (u32*)p;
-    }
-    *out = ptrs[device];
-    return hipSuccess;
-}
-
-hipError_t mk_k_derive_seeds(hipStream_t s, const u32* table, u32 w, const u32* seeds, u32 n, u32* out_pk) {
-    hipLaunchKernelGGL(mk_derive_seeds_kernel, dim3((n + 63) / 64), dim3(64), 0, s, table, w, seeds, n, out_pk);
+static hipError_t mk_k_derive_seeds(hipStream_t s, const u32* table, u32 w, const u32* seeds, u32 n, u32* out_pk) {
+    hipLaunchKernelGGL(MK_K(mk_derive_seeds_kernel), dim3((n + 63) / 64), dim3(64), 0, s, table, w, seeds, n, out_pk);
     return hipGetLastError();
 }
 
-hipError_t mk_k_reset(hipStream_t s, mk_counters* ctr, u32* stop, u32 nstop, const mk_serve_state* serve) {
-    hipLaunchKernelGGL(mk_reset_kernel, dim3(nstop > 64 ? (nstop + 63) / 64 : 1), dim3(64), 0, s, ctr, stop, nstop,
-                       serve);
-    return hipGetLastError();
-}
-
-hipError_t mk_k_search(hipStream_t s, u32 grid, u32 lock_waves, const u32* table, u32 w, u32 K,
+static hipError_t mk_k_search(hipStream_t s, u32 grid, u32 lock_waves, const u32* table, u32 w, u32 K,
                        const mk_search_desc* descs, const mk_seg* segs, u32 nsegs, u64 total_tiles, const u64* masks,
                        const u16* dfas, u32* spill, mk_counters* ctr, mk_cand* cands, u32 cand_cap, u32* stop,
                        const mk_serve_state* serve, u64* next_index) {
@@ -534,22 +504,91 @@ hipError_t mk_k_search(hipStream_t s, u32 grid, u32 lock_waves, const u32* table
     a.serve = serve;
     a.next_index = next_index;
     if (lock_waves == 0) {
-        hipLaunchKernelGGL(mk_search_kernel, dim3(grid), dim3(64), 0, s, a);
+        hipLaunchKernelGGL(MK_K(mk_search_kernel), dim3(grid), dim3(64), 0, s, a);
     } else {
         // grid waves in workgroups of lock_waves (4 or 8, dividing the grid)
         if ((lock_waves != 4 && lock_waves != 8) || grid % lock_waves != 0) return hipErrorInvalidValue;
         if (lock_waves == 8)
-            hipLaunchKernelGGL(mk_search_lock8_kernel, dim3(grid / 8), dim3(512), 0, s, a);
+            hipLaunchKernelGGL(MK_K(mk_search_lock8_kernel), dim3(grid / 8), dim3(512), 0, s, a);
         else
-            hipLaunchKernelGGL(mk_search_lock4_kernel, dim3(grid / 4), dim3(256), 0, s, a);
+            hipLaunchKernelGGL(MK_K(mk_search_lock4_kernel), dim3(grid / 4), dim3(256), 0, s, a);
     }
     return hipGetLastError();
 }
 
-hipError_t mk_k_confirm(hipStream_t s, const u32* table, u32 w, const mk_search_desc* descs, const u64* masks,
+static hipError_t mk_k_confirm(hipStream_t s, const u32* table, u32 w, const mk_search_desc* descs, const u64* masks,
                         const u16* dfas, mk_counters* ctr, const mk_cand* cands, u32 cand_cap, mk_hit* hits,
                         u32 hit_cap) {
-    hipLaunchKernelGGL(mk_confirm_kernel, dim3((cand_cap + 63) / 64), dim3(64), 0, s, table, w, descs, masks, dfas,
+    hipLaunchKernelGGL(MK_K(mk_confirm_kernel), dim3((cand_cap + 63) / 64), dim3(64), 0, s, table, w, descs, masks, dfas,
                        ctr, cands, cand_cap, hits, hit_cap);
     return hipGetLastError();
 }
+
+}  // namespace MK_NS
+
+#if !defined(__HIP_DEVICE_COMPILE__)  // host code only: the device pass has no launchers
+extern const mk_kernel_set MK_K(mk_kernels);
+const mk_kernel_set MK_K(mk_kernels) = {
+    MK_PACKED ? (MK_FIELD == 8 ? MK_LIMBS_R32 : MK_LIMBS_R25) : MK_LIMBS_R25U,
+    MK_ENTRY_WORDS,
+    (u32)sizeof(MK_NS::mk_ext),
+    MK_NS::mk_k_table_bases,
+    MK_NS::mk_k_table_fill,
+    MK_NS::mk_k_derive,
+    MK_NS::mk_k_derive_kv,
+    MK_NS::mk_k_derive_seeds,
+    MK_NS::mk_k_search,
+    MK_NS::mk_k_confirm,
+};
+#endif
+
+#ifdef MK_BASE_TU
+// -------------------------------------------- layout-independent kernels (one copy)
+
+// Staging of the allocation-free derivation (mk_derive_noalloc): MK_NOALLOC_KEYS public
+// keys, then as many seeds. File scope, one copy per device for the life of the process;
+// host code reaches it only through the address hipGetSymbolAddress gives per device.
+static __device__ __align__(256) u32 scratch[MK_NOALLOC_WORDS];
+
+// Clears the launch counters and the first nstop stop flags with a kernel (a plain
+// fill rather than a memset node, so the step can be captured in a graph).
+// With `serve`, the stop flags of the step's active slots are cleared instead.
+extern "C" __global__ void mk_reset_kernel(mk_counters* ctr, u32* stop, u32 nstop, const mk_serve_state* serve) {
+    const u32 i = blockIdx.x * 64 + threadIdx.x;
+    if (i == 0) {
+        ctr->queue = 0;
+        ctr->queue2 = 0;
+        ctr->n_cands = 0;
+        ctr->n_hits = 0;
+        ctr->overflow = 0;
+    }
+    if (serve != nullptr) {
+        if (i < serve->n_active && i < MK_SERVE_MAX_SLOTS) stop[serve->slots[i]] = 0;
+    } else if (stop != nullptr && i < nstop) {
+        stop[i] = 0;
+    }
+}
+
+// Device address of `scratch` on the current device, looked up once per device (each
+// device holds its own copy at its own address) and kept for the life of the process.
+hipError_t mk_k_noalloc_scratch(int device, u32** out) {
+    static std::mutex mu;
+    static u32* ptrs[MK_MAX_DEVICES] = {};
+    if (device < 0 || device >= MK_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (ptrs[device] == nullptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        ptrs[device] = (u32*)p;
+    }
+    *out = ptrs[device];
+    return hipSuccess;
+}
+
+hipError_t mk_k_reset(hipStream_t s, mk_counters* ctr, u32* stop, u32 nstop, const mk_serve_state* serve) {
+    hipLaunchKernelGGL(mk_reset_kernel, dim3(nstop > 64 ? (nstop + 63) / 64 : 1), dim3(64), 0, s, ctr, stop, nstop,
+                       serve);
+    return hipGetLastError();
+}
+#endif  // MK_BASE_TU

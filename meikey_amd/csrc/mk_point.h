@@ -125,12 +125,19 @@ struct mk_entry {
     u32 w[MK_ENTRY_WORDS];
 };
 
+// 16-byte chunks of one entry, as staged in LDS.
+#define MK_STAGE_CHUNKS (MK_ENTRY_WORDS / 4)
+
+#if !MK_PACKED
 // Word of limb i of field f (0 = y+x, 1 = y-x, 2 = third field) in a table entry. Limbs
 // 0-7 of the three fields fill 16-byte chunks 0-1, 2-3 and 4-5; limbs 8-9 of y+x and y-x
 // share chunk 6 and those of the third field start chunk 7 (2 pad words). y+x and y-x
 // thus sit at symmetric chunk positions, and a negative digit (which swaps them) only
 // changes which chunks a lane reads.
 MK_DEV int mk_entry_word(int f, int i) { return i < 8 ? 8 * f + i : 24 + 2 * f + (i - 8); }
+#endif
+// Packed entries: field f is the canonical value in words 8f .. 8f + 7 (chunks 2f, 2f + 1),
+// y+x and y-x again at mirrored chunk positions.
 
 #if defined(__HIPCC__)
 // One 128-byte entry: eight 16-byte vector loads of one cache line.
@@ -158,12 +165,18 @@ MK_DEV void mk_load_entry(const u32* table, u64 row, i32 d, u32 cols, mk_entry& 
 // and y-x, negate the third field).
 MK_DEV mk_niels mk_entry_niels(const mk_entry& e, i32 d) {
     mk_niels n;
+#if MK_PACKED
+    n.ypx = fe_from_words(&e.w[0]);
+    n.ymx = fe_from_words(&e.w[8]);
+    n.xyd = fe_from_words(&e.w[16]);
+#else
     MK_UNROLL
     for (int i = 0; i < 10; i++) {
         n.ypx.v[i] = e.w[mk_entry_word(0, i)];
         n.ymx.v[i] = e.w[mk_entry_word(1, i)];
         n.xyd.v[i] = e.w[mk_entry_word(2, i)];
     }
+#endif
     if (d < 0) {
         const fe t = n.ypx;
         n.ypx = n.ymx;
@@ -216,8 +229,9 @@ MK_DEV void mk_stage_entry(const u32* __restrict__ table, u64 row, i32 d, u32 co
     const u32* e = table + (row * cols + k) * MK_ENTRY_WORDS;
 #if defined(__HIP_DEVICE_COMPILE__)
     MK_UNROLL
-    for (int q = 0; q < MK_ENTRY_WORDS / 4; q++)
-        __builtin_amdgcn_global_load_lds((const void*)(e + 4 * q), (void*)(stage + (b * 8 + q) * 64), 16, 0, 0);
+    for (int q = 0; q < MK_STAGE_CHUNKS; q++)
+        __builtin_amdgcn_global_load_lds((const void*)(e + 4 * q), (void*)(stage + (b * MK_STAGE_CHUNKS + q) * 64),
+                                         16, 0, 0);
 #else
     (void)e;
     (void)stage;
@@ -225,11 +239,31 @@ MK_DEV void mk_stage_entry(const u32* __restrict__ table, u64 row, i32 d, u32 co
 #endif
 }
 
+#if MK_PACKED
+MK_DEV mk_niels mk_staged_niels(const uint4* stage, u32 b, u32 lane, i32 d) {
+    // chunk q of this lane: s[q * 64]. A negative digit reads y-x's chunks for y+x and
+    // the other way round, so the swap costs no moves; the words become limbs here.
+    const bool neg = d < 0;
+    const uint4* s = stage + b * MK_STAGE_CHUNKS * 64 + lane;
+    const uint4* pa = s + (neg ? 2 : 0) * 64;
+    const uint4* pb = s + (neg ? 0 : 2) * 64;
+    const uint4 a0 = pa[0], a1 = pa[64], b0 = pb[0], b1 = pb[64], c0 = s[4 * 64], c1 = s[5 * 64];
+    const u32 wa[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    const u32 wb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+    const u32 wc[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    mk_niels n;
+    n.ypx = fe_from_words(wa);
+    n.ymx = fe_from_words(wb);
+    n.xyd = fe_from_words(wc);
+    if (neg) n.xyd = fe_neg(n.xyd);
+    return n;
+}
+#else
 MK_DEV mk_niels mk_staged_niels(const uint4* stage, u32 b, u32 lane, i32 d) {
     // chunk q of this lane: s[q * 64]. A negative digit reads y-x's chunks for y+x and
     // the other way round, and the other half of chunk 6, so the swap costs no moves.
     const bool neg = d < 0;
-    const uint4* s = stage + b * 8 * 64 + lane;
+    const uint4* s = stage + b * MK_STAGE_CHUNKS * 64 + lane;
     const uint4* pa = s + (neg ? 2 : 0) * 64;
     const uint4* pb = s + (neg ? 0 : 2) * 64;
     const uint4 a0 = pa[0], a1 = pa[64], b0 = pb[0], b1 = pb[64], c0 = s[4 * 64], c1 = s[5 * 64];
@@ -249,6 +283,7 @@ MK_DEV mk_niels mk_staged_niels(const uint4* stage, u32 b, u32 lane, i32 d) {
     if (neg) n.xyd = fe_neg(n.xyd);
     return n;
 }
+#endif
 
 // mk_mul_base_frac with the gathers staged through LDS: the entry of row r + 1 is in
 // flight while row r is added. Every wave of the kernel calls this with all 64 lanes.
@@ -367,9 +402,9 @@ MK_DEV void mk_row_bases(u32 w, mk_ext* out) {
     }
 }
 
-// Entry (row, k) = k * base_row (+ 2^254 * B on the top row) as ((y+x)/2, (y-x)/2, dxy)
-// in canonical limbs, row 0 with xy as its third field; 32 words laid out by
-// mk_entry_word (2 pad words zero).
+// Entry (row, k) = k * base_row (+ 2^254 * B on the top row) as ((y+x)/2, (y-x)/2, dxy),
+// row 0 with xy as its third field, each field canonical: 32 words of limbs laid out by
+// mk_entry_word (2 pad words zero), or packed as 3 x 8 words.
 MK_DEV void mk_table_entry(u32 w, const mk_ext* bases, u32 row, u32 k, u32* out) {
     const u32 rows = mk_rows(w);
     const fe d2 = mk_const(MK_D2);
@@ -391,11 +426,17 @@ MK_DEV void mk_table_entry(u32 w, const mk_ext* bases, u32 row, u32 k, u32* out)
     for (int f = 0; f < 3; f++) {
         u32 wds[8];
         fe_carried_to_words(parts[f], wds);
+#if MK_PACKED
+        for (int i = 0; i < 8; i++) out[8 * f + i] = wds[i];
+#else
         const fe c = fe_from_words(wds);  // canonical limbs
         for (int i = 0; i < 10; i++) out[mk_entry_word(f, i)] = c.v[i];
+#endif
     }
+#if !MK_PACKED
     out[30] = 0;
     out[31] = 0;
+#endif
 }
 
 // s*B by textbook double-and-add over the 255 scalar bits with the unified addition:

@@ -173,9 +173,9 @@ mk_status enqueue_step(mk_server* s, u32 b) {
     step_buf& B = s->buf[b];
     MK_HIPS(hipMemcpyAsync(B.d_serve, B.h_serve, sizeof(mk_serve_state), hipMemcpyHostToDevice, B.stream));
     MK_HIPS(mk_k_reset(B.stream, B.ctr, s->d_stop, MK_SERVE_MAX_SLOTS, B.d_serve));
-    MK_HIPS(mk_k_search(B.stream, g->grid, g->lock_waves, g->table, g->opt.width, g->opt.batch, s->d_descs, nullptr, 0, s->step_tiles,
+    MK_HIPS(g->ks->search(B.stream, g->grid, g->lock_waves, g->table, g->opt.width, g->opt.batch, s->d_descs, nullptr, 0, s->step_tiles,
                         s->d_masks, s->d_dfas, B.spill, B.ctr, B.cands, kCandCap, s->d_stop, B.d_serve, s->d_next));
-    MK_HIPS(mk_k_confirm(B.stream, g->table, g->opt.width, s->d_descs, s->d_masks, s->d_dfas, B.ctr, B.cands, kCandCap,
+    MK_HIPS(g->ks->confirm(B.stream, g->table, g->opt.width, s->d_descs, s->d_masks, s->d_dfas, B.ctr, B.cands, kCandCap,
                          B.hits, kHitCap));
     MK_HIPS(hipMemcpyAsync(B.h_ctr, B.ctr, sizeof(mk_counters), hipMemcpyDeviceToHost, B.stream));
     MK_HIPS(hipMemcpyAsync(B.h_next, s->d_next, sizeof(u64) * s->opt.max_slots, hipMemcpyDeviceToHost, B.stream));

@@ -8,22 +8,22 @@
 
 // Per-lane spill area of the batched inversion: two field elements per key (the key's
 // numerator times the prefix product before it, and its denominator).
-// Element `i` of field slot `slot` lives at base[(slot * 10 + i) * stride]; with
+// Element `i` of field slot `slot` lives at base[(slot * MK_FE_LIMBS + i) * stride]; with
 // stride = number of lanes and base offset by the lane, the words of one slot are
 // contiguous across a wave.
 struct mk_spill {
     u32* base;
     u64 stride;
     MK_MEMBER void store(u32 slot, const fe& a) const {
-        u32* p = base + (u64)slot * 10 * stride;
+        u32* p = base + (u64)slot * MK_FE_LIMBS * stride;
         MK_UNROLL
-        for (int i = 0; i < 10; i++) p[(u64)i * stride] = a.v[i];
+        for (int i = 0; i < MK_FE_LIMBS; i++) p[(u64)i * stride] = a.v[i];
     }
     MK_MEMBER fe load(u32 slot) const {
-        const u32* p = base + (u64)slot * 10 * stride;
+        const u32* p = base + (u64)slot * MK_FE_LIMBS * stride;
         fe a;
         MK_UNROLL
-        for (int i = 0; i < 10; i++) a.v[i] = p[(u64)i * stride];
+        for (int i = 0; i < MK_FE_LIMBS; i++) a.v[i] = p[(u64)i * stride];
         return a;
     }
 };

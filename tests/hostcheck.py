@@ -10,39 +10,54 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PATH = os.path.join(ROOT, "meikey_amd", "libmeikey_hostcheck.so")
 
 _lib = None
+_layouts = {}
+
+
+def load(layout: str = ""):
+    """The CPU build of one layout: "" (ten limbs, 128-byte entries of limbs), "_p10" (ten
+    limbs, packed 96-byte entries) or "_r32" (eight 32-bit words, packed entries)."""
+    if layout not in _layouts:
+        path = os.path.join(ROOT, "meikey_amd", f"libmeikey_hostcheck{layout}.so")
+        if not os.path.exists(path):
+            from meikey_amd import build
+            build.build_hostcheck()
+        _layouts[layout] = _bind(ctypes.CDLL(path))
+    return _layouts[layout]
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(PATH):
-            from meikey_amd import build
-            build.build_hostcheck()
-        L = ctypes.CDLL(PATH)
-        P, U32, U64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
-        L.hc_seed.argtypes = [P, U32, U64, P]
-        L.hc_secret_scalar.argtypes = [P, P]
-        L.hc_secret_scalar_counter.argtypes = [P, U64, P]
-        L.hc_fe_mul_limbs.argtypes = [P, P, P]
-        L.hc_fe_sq_limbs.argtypes = [P, P]
-        L.hc_fe_to_words.argtypes = [P, P]
-        L.hc_fe_from_words.argtypes = [P, P]
-        L.hc_fe_op.argtypes = [U32, P, P, P]
-        L.hc_digits.argtypes = [P, U32, P]
-        L.hc_table_words.argtypes = [U32]
-        L.hc_table_words.restype = U64
-        L.hc_table_build.argtypes = [U32, P]
-        L.hc_table_entry.argtypes = [U32, U32, U32, P]
-        L.hc_full_pk.argtypes = [P, U32, P, P]
-        L.hc_lane.argtypes = [P, U32, P, U32, U64, U32, ctypes.c_char_p, U32, U32, U32, P, P]
-        L.hc_build_mask.argtypes = [ctypes.c_char_p, U32, U32, U32, P]
-        L.hc_match.argtypes = [P, ctypes.c_char_p, U32, U32, U32, U32]
-        L.hc_confirm.argtypes = [P, U32, U64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, U32, U32]
-        L.hc_pk_by_ladder.argtypes = [P, P]
-        L.hc_sort_hits.argtypes = [P, P, U32, P]
-        L.hc_fe_carried_to_words.argtypes = [P, P]
-        _lib = L
+        _lib = load("")
     return _lib
+
+
+def _bind(L):
+    P, U32, U64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
+    L.hc_seed.argtypes = [P, U32, U64, P]
+    L.hc_secret_scalar.argtypes = [P, P]
+    L.hc_secret_scalar_counter.argtypes = [P, U64, P]
+    L.hc_fe_mul_limbs.argtypes = [P, P, P]
+    L.hc_fe_sq_limbs.argtypes = [P, P]
+    L.hc_fe_to_words.argtypes = [P, P]
+    L.hc_fe_from_words.argtypes = [P, P]
+    L.hc_fe_op.argtypes = [U32, P, P, P]
+    L.hc_digits.argtypes = [P, U32, P]
+    L.hc_table_words.argtypes = [U32]
+    L.hc_table_words.restype = U64
+    L.hc_table_build.argtypes = [U32, P]
+    L.hc_table_entry.argtypes = [U32, U32, U32, P]
+    L.hc_full_pk.argtypes = [P, U32, P, P]
+    L.hc_lane.argtypes = [P, U32, P, U32, U64, U32, ctypes.c_char_p, U32, U32, U32, P, P]
+    L.hc_build_mask.argtypes = [ctypes.c_char_p, U32, U32, U32, P]
+    L.hc_match.argtypes = [P, ctypes.c_char_p, U32, U32, U32, U32]
+    L.hc_confirm.argtypes = [P, U32, U64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, U32, U32, U32]
+    L.hc_pk_by_ladder.argtypes = [P, P]
+    L.hc_sort_hits.argtypes = [P, P, U32, P]
+    L.hc_fe_carried_to_words.argtypes = [P, P]
+    L.hc_fe_op_limbs.argtypes = [U32, P, P, P]
+    L.hc_layout.restype = ctypes.c_int
+    return L
 
 
 def words(b: bytes):
