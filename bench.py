@@ -10,9 +10,11 @@ order of the reference's 200 ms launch target); the default 8 timed steps cover 
 Reports one JSON line: keys/s (value), the search kernel's fraction of the integer
 VALU multiply roofline (timed with HIP events on the library's stream), the 64-search
 batched launch against the single search and against one search per launch, the
-serving run (Poisson arrivals, p50/p99 time to first hit), and the CPU oracle's rate on
-the host cores over a bounded sample (which also checks the GPU's public keys of that
-sample, and every key the server returned, bit for bit).
+serving run (Poisson arrivals, p50/p99 time to first hit), and the CPU baseline: the
+restatement of the reference's SIMD CPU search (AVX-512 IFMA or AVX2, oracle/simd) timed
+for 10 s on pinned host threads, with the reference kernel's host build and the scalar C
+oracle beside it (the oracle also checks the GPU's public keys of its sample, and every
+key the server returned, bit for bit).
 """
 import argparse
 import ctypes
@@ -56,6 +58,7 @@ def main():
     ap.add_argument("--lockstep", type=int, default=None,
                     help="1: one workgroup per CU adding each comb row together (default: the library's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="timed length of the SIMD CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-search batched leg")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="serving run length (0 = skip)")
     ap.add_argument("--no-low", action="store_true", help="skip the low-security (counter seed) leg")
@@ -158,7 +161,7 @@ def main():
     if args.serve_seconds > 0:
         out["serving"], served = serving_leg(gpu, args.serve_seconds)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(gpu, key, needle, served, low)
+        out["cpu_baseline"] = cpu_baseline(gpu, key, needle, served, low, args.cpu_seconds)
     gpu.close()
     print(json.dumps(out), flush=True)
 
@@ -382,15 +385,56 @@ def reference_cpu_leg(key, needle, cores):
         g16.close()
 
 
-def cpu_baseline(gpu, key, needle, served=(), low=None):
+# Host threads for the CPU legs: every CPU of the affinity mask, at most the 16 CPUs that
+# a one-GPU share of the host provides.
+CPU_SHARE = 16
+
+
+def simd_cpu_leg(gpu, key, needle, seconds):
+    """The restatement of the reference's SIMD CPU search path (oracle/libmeikey_simd.so:
+    AVX-512 IFMA 5 x 51-bit lanes, or AVX2 where the CPU has no IFMA; 8-way SHA-512; W = 12
+    comb; 256-key batch inversion; unsigned-first matching), timed on pinned threads over
+    the headline workload. Its public keys of 2^16 sampled candidates are compared with the
+    GPU's first. None where the CPU has neither instruction set."""
+    from oracle import simd as S
+    try:
+        S.lib()
+    except RuntimeError as e:
+        print(f"cpu baseline: SIMD leg skipped ({e})", file=sys.stderr, flush=True)
+        return None
+    affinity = len(os.sched_getaffinity(0))
+    threads = max(1, min(affinity, CPU_SHARE))
+    model = S.cpu_model()
+    isa = {512: "AVX-512 IFMA", 2: "AVX2"}[S.isa()]
+    print(f"cpu baseline: {model}, {isa}, {threads} pinned threads ({affinity} CPUs in the affinity mask), "
+          f"{seconds:.0f} s", file=sys.stderr, flush=True)
+    checked = wrong = 0
+    for first in (0, 12345, (1 << 31) + 7, (1 << 32) - 4096):
+        n = 1 << 14
+        a, b = S.derive(key, 0, first, n), gpu.derive(key, 0, first, n)
+        checked += n
+        wrong += sum(1 for i in range(n) if a[32 * i:32 * i + 32] != b[32 * i:32 * i + 32])
+    r = S.bench(key, needle, True, 1, seconds, threads=threads)
+    return {"value": r["keys_per_s"], "unit": "keys/s", "cores": threads, "kind": "port",
+            "sample": f"{r['keys']} consecutive candidates of the headline key and suffix in {r['seconds']:.2f} s "
+                      f"on {threads} pinned threads, each from its own 2^40-aligned start (restatement of the "
+                      f"reference's SIMD CPU search, {isa} engine, W = 12 comb, 256-key inversion, unsigned-first "
+                      f"matching)",
+            "cpu_model": model, "isa": isa, "affinity_cpus": affinity,
+            "per_thread": r["keys_per_s"] / threads, "hits": r["hits"],
+            "simd_keys_checked_against_gpu": checked, "simd_keys_differing_from_gpu": wrong}
+
+
+def cpu_baseline(gpu, key, needle, served=(), low=None, simd_seconds=10.0):
     """CPU baselines on this host's cores over bounded samples of the same workload: the
-    reference's kernel compiled for the host (kind "reference", where oracle/_ref was
-    built) and the CPU oracle (a scalar C port, kind "port"). The GPU's public keys of
-    the oracle's sample, every key the server returned and (low-security leg) the keys of
-    every 2^16-th counter-seed index are checked against the oracle."""
+    restatement of the reference's SIMD CPU search (the line reported), the reference's
+    GPU kernel compiled for the host (where oracle/_ref was built) and the scalar C
+    oracle. The GPU's public keys of the oracle's sample, every key the server returned
+    and (low-security leg) the keys of every 2^16-th counter-seed index are checked
+    against the oracle."""
     from oracle import oracle as O
 
-    cores = min(os.cpu_count() or 1, 16)
+    cores = max(1, min(len(os.sched_getaffinity(0)), CPU_SHARE))
     chunks, per = 16, 1 << 16  # 2^20 keys spread over the 2^32 range
     firsts = [c * (1 << 28) + 12345 for c in range(chunks)]
     t0 = time.perf_counter()
@@ -425,12 +469,14 @@ def cpu_baseline(gpu, key, needle, served=(), low=None):
     checks = {"served_keys_checked": len(served), "served_keys_wrong": bad_served,
               "low_security_keys_checked": low_checked, "low_security_keys_wrong": low_wrong,
               "checked_keys": chunks * per, "mismatched_chunks": mismatches}
-    # the reference's own kernel (host build) is the baseline where it was built; the
-    # port's rate is reported beside it
+    # the SIMD restatement of the reference's CPU path is the baseline; the reference
+    # kernel's host build and the scalar oracle are reported beside it
+    simd = simd_cpu_leg(gpu, key, needle, simd_seconds)
     ref = reference_cpu_leg(key, needle, cores)
-    out = dict(ref) if ref else dict(port)
-    if ref:
-        out["port"] = port
+    out = dict(simd) if simd else dict(ref) if ref else dict(port)
+    if simd and ref:
+        out["reference_kernel_host_build"] = ref
+    out["scalar_oracle"] = port
     out.update(checks)
     return out
 

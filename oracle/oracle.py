@@ -18,7 +18,8 @@ CONTAINS, SUFFIX, REGEX = 0, 1, 2
 
 
 def build(ref: bool = True) -> None:
-    """Compile liboracle.so, and the reference host build where its sources exist."""
+    """Compile liboracle.so and the SIMD CPU baseline (libmeikey_simd.so), and the
+    reference host build where its sources exist."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     if ref and os.path.exists("/root/reference/src/service/gpu/host.cpp"):
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
