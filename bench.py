@@ -36,12 +36,15 @@ ctypes_u32, ctypes_u64, ctypes_buf = ctypes.c_uint32, ctypes.c_uint64, ctypes.cr
 PEAK_MUL = 256 * 4 * 16 * 2.4e9
 
 
-def field_mults_per_key(w: int, k: int) -> float:
+def field_mults_per_key(w: int, k: int, pool: int = 0) -> float:
     """Algorithmic field multiplications per key: 7 per extended+affine addition over
     rows - 2 rows, 3 for the last (fraction) row, 3 for the batched inversion plus 1 for
-    the prefix product, and one 265-multiplication inversion shared by k keys."""
+    the prefix product, and one 265-multiplication inversion shared by k keys (with a
+    pooled inversion, by the pool x k keys of a lane's chain, which adds 3 (pool - 1)
+    multiplications)."""
     rows = -(-251 // w)
-    return 7 * (rows - 2) + 3 + 4 + 265.0 / k
+    inv = 265.0 / k if not pool else (265.0 + 3 * (pool - 1)) / (pool * k)
+    return 7 * (rows - 2) + 3 + 4 + inv
 
 
 def main():
@@ -55,6 +58,8 @@ def main():
     ap.add_argument("--limbs", type=int, default=None,
                     help="field / table layout: 10 (packed ten-limb), 11 (ten-limb entries), 8 (eight words); "
                          "default: the library's")
+    ap.add_argument("--pool", type=int, default=None,
+                    help="4 or 8: workgroups of that many waves share one inversion (default: the library's)")
     ap.add_argument("--lockstep", type=int, default=None,
                     help="1: one workgroup per CU adding each comb row together (default: the library's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -80,9 +85,12 @@ def main():
     needle = needle.encode()
     if args.lockstep is None:
         args.lockstep = M.default_options().lockstep
+    if args.pool is None:
+        args.pool = M.default_options().pool
     if args.limbs is None:
         args.limbs = M.default_options().limbs
-    gpu = M.Gpu(0, width=args.width, batch=args.batch, blocks=args.blocks, limbs=args.limbs, lockstep=args.lockstep)
+    gpu = M.Gpu(0, width=args.width, batch=args.batch, blocks=args.blocks, limbs=args.limbs, lockstep=args.lockstep,
+                pool=args.pool)
     info = gpu.info()
     print(f"# device {info.name.decode()} CUs={info.compute_units} grid={info.grid_waves} waves "
           f"W={args.width} batch={args.batch} table={info.table_bytes / 2**20:.1f} MiB needle={needle.decode()}",
@@ -111,7 +119,7 @@ def main():
     # the run uses, so the achieved rate is proportional to keys/s; the run's own count
     # (fewer multiplications at larger widths) is reported beside it.
     mults = field_mults_per_key(16, 256)
-    run_mults = field_mults_per_key(args.width, args.batch)
+    run_mults = field_mults_per_key(args.width, args.batch, args.pool)
     achieved = STEP_KEYS * mults * 64 / (avg_ms * 1e-3)
     achieved_run = STEP_KEYS * run_mults * 64 / (avg_ms * 1e-3)
     traffic = measured_traffic(args.width, args.batch, args.blocks)
@@ -127,7 +135,7 @@ def main():
                                "(random.Random(2)), 6-char case-sensitive suffix",
                    "candidates": keys, "needle": needle.decode(), "width": args.width,
                    "keys_per_thread": args.batch, "waves_per_simd": args.blocks, "grid_waves": info.grid_waves,
-                   "lockstep": args.lockstep, "limbs": args.limbs,
+                   "lockstep": args.lockstep, "pool": args.pool, "limbs": args.limbs,
                    "hits": len(all_hits)},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_MUL / 1e12, "unit": "TOP/s",
                      "frac": achieved / PEAK_MUL, "traffic": traffic,

@@ -69,6 +69,10 @@ typedef struct {
     uint32_t lockstep; /* 0: one wave per workgroup; 1: the waves of a CU form one workgroup and add each
                           comb row together, which keeps the CU's address translation on one or two rows
                           of the table (blocks 1..2); 2: the same with workgroups of one wave per SIMD */
+    uint32_t pool;     /* 0: each lane inverts its own batch; 4 or 8: workgroups of that many waves share
+                          one inversion (lane l of wave 0 inverts the product of lane l's batches of all
+                          the waves; the reference's block-pooled inversion, kernel.cu:1529-1541).
+                          8 needs blocks 2, 4 needs blocks 1..2; not with lockstep */
 } mk_options;
 
 /* Device status (gpu.rs:1251-1326 status rows). */
@@ -323,8 +327,10 @@ typedef struct {
 
 mk_status mk_fastest(int ordinal, const mk_options* candidates, uint32_t n, mk_tune_result* results, mk_options* best);
 /* Stage 1: the layouts "10u", "10" and "8" at W = 16 (gpu.rs:364-370); stage 2: widths
- * 16/20/22/24; stage 3: 32/64/128/256 keys per lane; stage 4: 1 or 2 waves per SIMD, and
- * lockstep workgroups. results (cap entries) receive every measured setting; *n the count. */
+ * 16/20/22/24; stage 3: the inversion, 32/64/128/256 keys per lane each inverting its own
+ * batch or 8/32 keys per lane with one inversion per 8-wave workgroup (pool 8); stage 4:
+ * 1 or 2 waves per SIMD, and lockstep workgroups (for a per-lane inversion). results (cap
+ * entries) receive every measured setting; *n the count. */
 mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* results, uint32_t cap, uint32_t* n);
 
 /* OpenSSH text of a key: the authorized_keys line "ssh-ed25519 <b64>[ <comment>]"

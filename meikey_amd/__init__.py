@@ -62,7 +62,7 @@ class MeikeyError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("batch", ctypes.c_uint32), ("limbs", ctypes.c_uint32),
-                ("blocks", ctypes.c_uint32), ("lockstep", ctypes.c_uint32)]
+                ("blocks", ctypes.c_uint32), ("lockstep", ctypes.c_uint32), ("pool", ctypes.c_uint32)]
 
 
 class Info(ctypes.Structure):
@@ -230,13 +230,15 @@ def openssh_private(seed: bytes, pk: bytes, comment: str = "", checkint: Optiona
 
 
 def fastest(ordinal: int, candidates):
-    """Measure each (width, batch, blocks[, limbs[, lockstep]]) candidate: (best Options,
-    rows as fastest_staged). Limbs default to the library's, lockstep to 0."""
+    """Measure each (width, batch, blocks[, limbs[, lockstep[, pool]]]) candidate: (best
+    Options, rows as fastest_staged). Limbs default to the library's, lockstep and pool
+    to 0."""
     lim = default_options().limbs
     opts = []
     for c in candidates:
         w, b, k = c[:3]
-        opts.append(Options(w, b, c[3] if len(c) > 3 else lim, k, c[4] if len(c) > 4 else 0))
+        opts.append(Options(w, b, c[3] if len(c) > 3 else lim, k, c[4] if len(c) > 4 else 0,
+                            c[5] if len(c) > 5 else 0))
     cands = (Options * len(opts))(*opts)
     res = (TuneResult * len(candidates))()
     best = Options()
@@ -246,13 +248,14 @@ def fastest(ordinal: int, candidates):
 
 
 def fastest_staged(ordinal: int = 0):
-    """The staged autotuner: (best Options, [(width, batch, blocks, keys/s, status), ...])."""
+    """The staged autotuner: (best Options, [(width, batch, blocks, keys/s, status, limbs,
+    lockstep, pool), ...])."""
     best = Options()
     res = (TuneResult * 64)()
     n = ctypes.c_uint32(0)
     _check(lib().mk_fastest_staged(ordinal, ctypes.byref(best), res, 64, ctypes.byref(n)))
     rows = [(r.options.width, r.options.batch, r.options.blocks, r.keys_per_s, r.status, r.options.limbs,
-             r.options.lockstep) for r in res[:n.value]]
+             r.options.lockstep, r.options.pool) for r in res[:n.value]]
     return best, rows
 
 
@@ -399,10 +402,10 @@ class Gpu:
     """One device with its comb table (the reference's Gpu + Kernel)."""
 
     def __init__(self, ordinal: int = 0, width: int = 24, batch: int = 256, blocks: int = 2,
-                 limbs: Optional[int] = None, lockstep: int = 0):
+                 limbs: Optional[int] = None, lockstep: int = 0, pool: int = 0):
         if limbs is None:
             limbs = default_options().limbs
-        self.options = Options(width, batch, limbs, blocks, lockstep)
+        self.options = Options(width, batch, limbs, blocks, lockstep, pool)
         h = ctypes.c_void_p()
         _check(lib().mk_open(ordinal, ctypes.byref(self.options), ctypes.byref(h)))
         self._h = h

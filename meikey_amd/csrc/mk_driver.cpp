@@ -295,12 +295,22 @@ mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* resul
     }
     mk_options w1 = w0;
     if (st == MK_OK) st = stage(c1, w1);
-    // stage 3: keys per lane (inversion batch) at the best width
+    // stage 3: the inversion at the best width: keys per lane, each lane inverting its own
+    // batch, or one inversion shared by the 8 waves of a workgroup (8 keys per lane, the
+    // reference's pooled shape, and 32)
     std::vector<mk_options> c2;
     for (uint32_t b : {32u, 64u, 128u, 256u}) {
         if (b == w1.batch) continue;
         mk_options o = w1;
         o.batch = b;
+        c2.push_back(o);
+    }
+    for (uint32_t b : {8u, 32u}) {
+        mk_options o = w1;
+        o.batch = b;
+        o.blocks = 2;
+        o.lockstep = 0;
+        o.pool = 8;
         c2.push_back(o);
     }
     mk_options w2 = w1;
@@ -309,11 +319,18 @@ mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* resul
         st = stage(c2, w2);
     }
     // stage 4: resident waves per SIMD, and workgroups that add each comb row in lockstep
+    // (a pooled winner keeps 2 waves per SIMD and tries its 4-wave workgroups instead)
     mk_options w3 = w2;
     if (st == MK_OK) {
         mk_options o = w2, l = w2;
-        o.blocks = w2.blocks == 2 ? 1 : 2;
-        l.lockstep = 2;
+        if (w2.pool != 0) {
+            o.pool = 4;
+            l.pool = 4;
+            l.blocks = 1;
+        } else {
+            o.blocks = w2.blocks == 2 ? 1 : 2;
+            l.lockstep = 2;
+        }
         st = stage({w2, o, l}, w3);
     }
     if (n_out) *n_out = (uint32_t)all.size();

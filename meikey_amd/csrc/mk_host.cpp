@@ -232,6 +232,7 @@ void mk_default_options(mk_options* out) {
     out->limbs = MK_LIMBS_R25U;
     out->blocks = 2;
     out->lockstep = 0;
+    out->pool = 0;
 }
 
 mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
@@ -256,6 +257,12 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
         return fail(MK_ERR_OUT_OF_RANGE,
                     "lockstep is 0, 1 (one workgroup per CU, 1 or 2 waves per SIMD) or 2 (workgroups of one wave per "
                     "SIMD) (lockstep %u, blocks %u)", opt.lockstep, opt.blocks);
+    if ((opt.pool != 0 && opt.pool != 4 && opt.pool != 8) || (opt.pool == 8 && opt.blocks != 2) ||
+        (opt.pool == 4 && opt.blocks > 2) || (opt.pool != 0 && opt.lockstep != 0))
+        return fail(MK_ERR_OUT_OF_RANGE,
+                    "pool is 0, 4 (workgroups of one wave per SIMD, 1 or 2 waves per SIMD) or 8 (one workgroup per "
+                    "CU, 2 waves per SIMD), and not combined with lockstep (pool %u, blocks %u, lockstep %u)",
+                    opt.pool, opt.blocks, opt.lockstep);
     int count = 0;
     mk_status st = mk_device_count(&count);
     if (st != MK_OK) return st;
@@ -285,8 +292,12 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
     g->cus = (u32)prop.multiProcessorCount;
     g->hbm = prop.totalGlobalMem;
     g->grid = g->cus * 4 * opt.blocks;
-    // lockstep 1: one workgroup per CU; 2: workgroups of one wave per SIMD
-    g->lock_waves = opt.lockstep == 1 ? 4 * opt.blocks : opt.lockstep == 2 ? 4 : 0;
+    // lockstep 1: one workgroup per CU; 2: workgroups of one wave per SIMD; pool: workgroups
+    // of `pool` waves sharing an inversion
+    g->lock_waves = opt.pool != 0 ? (MK_WG_POOL | opt.pool)
+                    : opt.lockstep == 1 ? 4 * opt.blocks
+                    : opt.lockstep == 2 ? 4
+                                        : 0;
     g->lanes = (u64)g->grid * 64;
     g->rows = rows_of(opt.width);
     g->cols = cols_of(opt.width);

@@ -186,10 +186,11 @@ __device__ __forceinline__ u32 mk_units_per_tile(u32 K, bool serving) {
 // One claim of one wave: tile `tile` (sub-unit `sub` of a tail tile) with Keff keys per
 // lane; nth is the claim's number in claim order (serving deals claims round-robin over
 // the active requests). Returns false, having searched nothing, when the claim's search
-// has stopped; otherwise the comb multiply calls row_sync() Keff x (rows - 2) times.
-template <class R>
+// has stopped; otherwise the comb multiply calls row_sync() Keff x (rows - 2) times and
+// the batched inversion calls invert(P) once.
+template <class R, class I>
 __device__ __forceinline__ bool mk_search_claim(const mk_search_args& a, mk_wave_ctx& cx, u64 tile, u64 sub,
-                                                u32 Keff, u64 nth, R row_sync) {
+                                                u32 Keff, u64 nth, R row_sync, I invert) {
     const u32 lane = cx.lane;
     u32 search;
     u64 tile_first;
@@ -305,9 +306,14 @@ __device__ __forceinline__ bool mk_search_claim(const mk_search_args& a, mk_wave
                 atomicOr(&ctr->overflow, 1u);
             }
         }
-    }, mid);
+    }, mid, invert);
     return true;
 }
+
+// The lane's own inversion (one per lane per claim).
+struct mk_lane_invert {
+    __device__ __forceinline__ fe operator()(const fe& p) const { return fe_invert(p); }
+};
 
 // The search kernel: one wave per workgroup, each wave claims its own tiles.
 extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
@@ -341,8 +347,55 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
         if (a.serve != nullptr && a.serve->n_active == 0) break;
         // claim number: whole tiles first, then the units of the tail
         const u64 nth = tile < big_tiles ? tile : big_tiles + (tile - big_tiles) * U + sub;
-        mk_search_claim(a, cx, tile, sub, Keff, nth, [] {});
+        mk_search_claim(a, cx, tile, sub, Keff, nth, [] {}, mk_lane_invert());
     }
+}
+
+// The inversion shared by the NW waves of a workgroup (the reference's block-pooled
+// inversion, kernel.cu:1529-1541, 2024-2028): lane l of every wave puts its product P in
+// LDS; lane l of wave 0 inverts the product of the NW values of lane l (Montgomery:
+// NW - 1 prefix products, one inversion, 2 (NW - 1) multiplications back) and hands each
+// wave its own 1/P. One inversion's instruction stream per workgroup instead of one per
+// wave; the other waves wait at the barrier, and their SIMDs run the other resident
+// waves meanwhile. Values live in the waves' staging buffers, which are idle between a
+// key's last comb row and the next key's first (every copy into them has been waited
+// for). Every wave of the workgroup calls this once per claim round, a wave with nothing
+// to search with P = 1; two barriers.
+template <u32 NW>
+__device__ __forceinline__ fe mk_pool_invert(const fe& p, uint4 (*stage)[2 * MK_STAGE_CHUNKS * 64], u32 wave,
+                                             u32 lane) {
+    auto put = [&](u32 w, u32 slot, const fe& v) {
+        u32* r = (u32*)stage[w] + slot * MK_FE_LIMBS * 64 + lane;
+        MK_UNROLL
+        for (int i = 0; i < MK_FE_LIMBS; i++) r[i * 64] = v.v[i];
+    };
+    auto get = [&](u32 w, u32 slot) {
+        const u32* r = (const u32*)stage[w] + slot * MK_FE_LIMBS * 64 + lane;
+        fe v;
+        MK_UNROLL
+        for (int i = 0; i < MK_FE_LIMBS; i++) v.v[i] = r[i * 64];
+        return v;
+    };
+    put(wave, 0, p);
+    __syncthreads();
+    if (wave == 0) {
+        // slot 1 of wave w: the prefix product P_0 ... P_w
+        fe acc = get(0, 0);
+        put(0, 1, acc);
+        for (u32 w = 1; w < NW; w++) {
+            acc = fe_mul(get(w, 0), acc);
+            put(w, 1, acc);
+        }
+        fe inv = fe_invert(acc);
+        for (u32 w = NW - 1; w > 0; w--) {
+            const fe q = get(w, 0);
+            put(w, 0, fe_mul(get(w - 1, 1), inv));  // 1 / P_w
+            inv = fe_mul(q, inv);                     // 1 / (P_0 ... P_(w-1))
+        }
+        put(0, 0, inv);
+    }
+    __syncthreads();
+    return get(wave, 0);
 }
 
 // The lockstep search kernel: one workgroup of 4 x (waves per SIMD) waves per CU. The
@@ -355,7 +408,10 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
 // same barrier sequence. Every branch that decides how many barriers a wave passes
 // (the claim, the end of the queue, an empty serving step) reads values that are equal
 // for the whole workgroup.
-template <u32 NW>
+//
+// With POOL the waves share one inversion per claim round (mk_pool_invert); without ROWS
+// there is no barrier per comb row.
+template <u32 NW, bool ROWS, bool POOL>
 __device__ __forceinline__ void mk_search_lock(const mk_search_args& a) {
     __shared__ u64 lmask[NW][64];
     __shared__ uint4 stage[NW][2 * MK_STAGE_CHUNKS * 64];
@@ -396,10 +452,20 @@ __device__ __forceinline__ void mk_search_lock(const mk_search_args& a) {
         const u64 tile = whole ? c : big_tiles + (c - big_tiles) / U;
         const u64 sub = whole ? 0 : (c - big_tiles) % U;
         bool searched = false;
+        auto row_sync = [] {
+            if constexpr (ROWS) __builtin_amdgcn_s_barrier();
+        };
+        auto invert = [&](const fe& p) {
+            if constexpr (POOL) return mk_pool_invert<NW>(p, stage, wave, cx.lane);
+            else return fe_invert(p);
+        };
         if (c < total_claims && (!whole || c < big_tiles))
-            searched = mk_search_claim(a, cx, tile, sub, Keff, c, [] { __builtin_amdgcn_s_barrier(); });
-        if (!searched)
-            for (u32 j = 0; j < Keff * (rows - 2); j++) __builtin_amdgcn_s_barrier();
+            searched = mk_search_claim(a, cx, tile, sub, Keff, c, row_sync, invert);
+        if (!searched) {
+            if constexpr (ROWS)
+                for (u32 j = 0; j < Keff * (rows - 2); j++) __builtin_amdgcn_s_barrier();
+            if constexpr (POOL) (void)mk_pool_invert<NW>(fe_small(1), stage, wave, cx.lane);
+        }
     }
 }
 
@@ -407,11 +473,20 @@ __device__ __forceinline__ void mk_search_lock(const mk_search_args& a) {
 // per SIMD, two workgroups share a CU and step through the rows independently).
 extern "C" __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
     MK_K(mk_search_lock8_kernel)(mk_search_args a) {
-    mk_search_lock<8>(a);
+    mk_search_lock<8, true, false>(a);
 }
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
     MK_K(mk_search_lock4_kernel)(mk_search_args a) {
-    mk_search_lock<4>(a);
+    mk_search_lock<4, true, false>(a);
+}
+// The pooled inversion over workgroups of 8 and of 4 waves, without row barriers.
+extern "C" __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
+    MK_K(mk_search_pool8_kernel)(mk_search_args a) {
+    mk_search_lock<8, false, true>(a);
+}
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
+    MK_K(mk_search_pool4_kernel)(mk_search_args a) {
+    mk_search_lock<4, false, true>(a);
 }
 // Exact check of every candidate: full key with its x sign, matched with the sign
 // known. One thread per candidate slot.
@@ -506,9 +581,15 @@ static hipError_t mk_k_search(hipStream_t s, u32 grid, u32 lock_waves, const u32
     if (lock_waves == 0) {
         hipLaunchKernelGGL(MK_K(mk_search_kernel), dim3(grid), dim3(64), 0, s, a);
     } else {
-        // grid waves in workgroups of lock_waves (4 or 8, dividing the grid)
-        if ((lock_waves != 4 && lock_waves != 8) || grid % lock_waves != 0) return hipErrorInvalidValue;
-        if (lock_waves == 8)
+        // grid waves in workgroups of nw (4 or 8, dividing the grid)
+        const u32 nw = lock_waves & ~MK_WG_POOL;
+        const bool pool = (lock_waves & MK_WG_POOL) != 0;
+        if ((nw != 4 && nw != 8) || grid % nw != 0) return hipErrorInvalidValue;
+        if (pool && nw == 8)
+            hipLaunchKernelGGL(MK_K(mk_search_pool8_kernel), dim3(grid / 8), dim3(512), 0, s, a);
+        else if (pool)
+            hipLaunchKernelGGL(MK_K(mk_search_pool4_kernel), dim3(grid / 4), dim3(256), 0, s, a);
+        else if (nw == 8)
             hipLaunchKernelGGL(MK_K(mk_search_lock8_kernel), dim3(grid / 8), dim3(512), 0, s, a);
         else
             hipLaunchKernelGGL(MK_K(mk_search_lock4_kernel), dim3(grid / 4), dim3(256), 0, s, a);

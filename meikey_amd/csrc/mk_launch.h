@@ -8,6 +8,9 @@
 #include "../../include/meikey_hip.h"
 #include "mk_kernels.h"
 
+// mk_kernel_set::search lock_waves flag: workgroups of 4 or 8 waves sharing one inversion.
+#define MK_WG_POOL 0x100u
+
 struct mk_kernel_set {
     u32 limbs;        // mk_options.limbs code of this layout
     u32 entry_words;  // 32-bit words per comb-table entry
@@ -19,8 +22,9 @@ struct mk_kernel_set {
     hipError_t (*derive_kv)(hipStream_t s, const u32* table, u32 w, const mk_key_words& key, u32 source, u64 first,
                             u64 n, u32* out_pk, u32* out_seed);
     hipError_t (*derive_seeds)(hipStream_t s, const u32* table, u32 w, const u32* seeds, u32 n, u32* out_pk);
-    // The search kernel over `grid` waves: one wave per workgroup (lock_waves = 0), or the
-    // lockstep variant in workgroups of lock_waves waves (4 or 8; grid a multiple of it).
+    // The search kernel over `grid` waves: one wave per workgroup (lock_waves = 0), the
+    // lockstep variant in workgroups of lock_waves waves (4 or 8; grid a multiple of it),
+    // or, with MK_WG_POOL set, workgroups of 4 or 8 waves that share one inversion.
     hipError_t (*search)(hipStream_t s, u32 grid, u32 lock_waves, const u32* table, u32 w, u32 K,
                          const mk_search_desc* descs, const mk_seg* segs, u32 nsegs, u64 total_tiles,
                          const u64* masks, const u16* dfas, u32* spill, mk_counters* ctr, mk_cand* cands,

@@ -29,8 +29,10 @@ struct mk_spill {
 };
 
 // Keys first .. first + K - 1: seed -> scalar -> (yn, yd) by mul(s, yn, yd); one
-// inversion of the product of the yd; then pre_sweep(); then, from the last key to the
-// first, y = yn / yd is compressed (x sign bit left 0) and on_key(j, pk_words) is called.
+// inversion of the product of the yd, by invert(P) (fe_invert, or an inversion shared
+// with other lanes that returns this lane's 1/P); then pre_sweep(); then, from the last
+// key to the first, y = yn / yd is compressed (x sign bit left 0) and on_key(j,
+// pk_words) is called.
 // Counter seeds (source 1) hash through the search's SHA-512 mid-state, which mid()
 // returns; the search kernel keeps it in LDS and reads it afresh for each key, so its
 // 22 registers are not held through the comb multiply.
@@ -41,9 +43,9 @@ struct mk_spill {
 // 1/P_(j-1) by a multiplication with yd_j. Four multiplications per key, as with the
 // textbook form that spills yn, yd and P_(j-1), but two spilled fields instead of three and
 // one multiplication between a key's loads and its y.
-template <class M, class P, class F, class G>
+template <class M, class P, class F, class G, class I>
 MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const mk_spill& sp, M mul, P pre_sweep,
-                         F on_key, G mid) {
+                         F on_key, G mid, I invert) {
     u32 upper[8];
     fe pre = fe_small(1);  // P_(j-1); each yd and yn is a first operand of fe_mul
     const u64 tail = (u64)key[7] << 32 | key[6];
@@ -64,7 +66,7 @@ MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const m
         sp.store(2 * j + 1, yd);
         pre = fe_mul(yd, pre);
     }
-    fe inv = fe_invert(pre);  // 1 / P_(K-1)
+    fe inv = invert(pre);  // 1 / P_(K-1)
     pre_sweep();
     // Each key's two spilled values are loaded one key ahead, so their HBM latency
     // overlaps the previous key's multiplications and matching.
@@ -81,4 +83,10 @@ MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const m
         if (jj > 0) inv = fe_mul(yd, inv);          // 1 / P_(jj-1)
         on_key(jj, pk);
     }
+}
+
+template <class M, class P, class F, class G>
+MK_DEV void mk_lane_tile(u32 K, const u32 key[8], u32 source, u64 first, const mk_spill& sp, M mul, P pre_sweep,
+                         F on_key, G mid) {
+    mk_lane_tile(K, key, source, first, sp, mul, pre_sweep, on_key, mid, [](const fe& p) { return fe_invert(p); });
 }
