@@ -212,8 +212,9 @@ def batched_leg(gpu, single_rate):
 def low_security_leg(gpu, args):
     """Low-security mode: counter seeds of the master key of random.Random(3), an 8-char
     case-insensitive substring, 2^32 candidates in 8 launches. Beside it the same search
-    with a cache-resident comb (W = 6: 42 rows x 33 entries x 128 B = 177 KB, the size
-    class of an LDS-resident table) to show the table-size / field-work trade."""
+    with a cache-resident comb (W = 6: 42 rows x 33 entries x 128 B = 177 KB, gathered
+    through L2) and with the comb table in LDS (W = 5 and W = 6), to show the table-size /
+    field-work trade."""
     import random
     import meikey_amd as M
     from meikey_amd import workloads as W
@@ -237,12 +238,24 @@ def low_security_leg(gpu, args):
     small_bytes = small.info().table_bytes
     small_rate, small_kernel, _ = run(small, 2, 1 << 26)
     small.close()
+    # the comb table in LDS (each CU's copy, read without HBM / L2 gathers): W = 5 with
+    # 128-byte entries and W = 6 with 96-byte entries, 32 keys per lane
+    lds = {}
+    for w, limbs in ((5, M.LIMBS_R25U), (6, M.LIMBS_R25)):
+        g = M.Gpu(0, width=w, batch=32, blocks=2, limbs=limbs, lds_table=1)
+        lds_bytes = g.info().table_bytes
+        r, kr, _ = run(g, 2, 1 << 26)
+        g.close()
+        lds[f"w{w}"] = {"keys_per_s": r, "kernel_keys_per_s": kr, "table_bytes": lds_bytes, "limbs": limbs,
+                        "candidates": 2 << 26}
     print(f"# low-security: {rate / 1e9:.3f} G keys/s (W={args.width}), {small_rate / 1e9:.3f} G keys/s (W=6 "
-          f"cache-resident table), {len(hits)} hits", file=sys.stderr, flush=True)
+          f"cache-resident table), {lds['w5']['keys_per_s'] / 1e9:.3f} / {lds['w6']['keys_per_s'] / 1e9:.3f} G keys/s "
+          f"(W=5 / W=6 table in LDS), {len(hits)} hits", file=sys.stderr, flush=True)
     return ({"value": rate, "unit": "keys/s", "candidates": launches * STEP_KEYS, "needle": needle.decode(),
              "kernel_keys_per_s": kernel_rate, "hits": len(hits), "vs_secure": None,
              "w6_table": {"keys_per_s": small_rate, "kernel_keys_per_s": small_kernel, "table_bytes": small_bytes,
-                          "candidates": 2 << 26}},
+                          "candidates": 2 << 26},
+             "lds_table": lds},
             (key, needle))
 
 

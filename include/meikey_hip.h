@@ -73,6 +73,9 @@ typedef struct {
                           one inversion (lane l of wave 0 inverts the product of lane l's batches of all
                           the waves; the reference's block-pooled inversion, kernel.cu:1529-1541).
                           8 needs blocks 2, 4 needs blocks 1..2; not with lockstep */
+    uint32_t lds_table; /* 1: each CU copies the whole comb table into LDS and its 8 waves read it there
+                           (tables up to 144 KB: W <= 5, or W = 6 with limbs 10 or 8); blocks 2, no
+                           lockstep or pool. 0: entries gathered from HBM / L2 */
 } mk_options;
 
 /* Device status (gpu.rs:1251-1326 status rows). */

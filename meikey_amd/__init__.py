@@ -62,7 +62,8 @@ class MeikeyError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("batch", ctypes.c_uint32), ("limbs", ctypes.c_uint32),
-                ("blocks", ctypes.c_uint32), ("lockstep", ctypes.c_uint32), ("pool", ctypes.c_uint32)]
+                ("blocks", ctypes.c_uint32), ("lockstep", ctypes.c_uint32), ("pool", ctypes.c_uint32),
+                ("lds_table", ctypes.c_uint32)]
 
 
 class Info(ctypes.Structure):
@@ -402,10 +403,10 @@ class Gpu:
     """One device with its comb table (the reference's Gpu + Kernel)."""
 
     def __init__(self, ordinal: int = 0, width: int = 24, batch: int = 256, blocks: int = 2,
-                 limbs: Optional[int] = None, lockstep: int = 0, pool: int = 0):
+                 limbs: Optional[int] = None, lockstep: int = 0, pool: int = 0, lds_table: int = 0):
         if limbs is None:
             limbs = default_options().limbs
-        self.options = Options(width, batch, limbs, blocks, lockstep, pool)
+        self.options = Options(width, batch, limbs, blocks, lockstep, pool, lds_table)
         h = ctypes.c_void_p()
         _check(lib().mk_open(ordinal, ctypes.byref(self.options), ctypes.byref(h)))
         self._h = h

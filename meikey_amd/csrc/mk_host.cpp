@@ -233,6 +233,7 @@ void mk_default_options(mk_options* out) {
     out->blocks = 2;
     out->lockstep = 0;
     out->pool = 0;
+    out->lds_table = 0;
 }
 
 mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
@@ -263,6 +264,14 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
                     "pool is 0, 4 (workgroups of one wave per SIMD, 1 or 2 waves per SIMD) or 8 (one workgroup per "
                     "CU, 2 waves per SIMD), and not combined with lockstep (pool %u, blocks %u, lockstep %u)",
                     opt.pool, opt.blocks, opt.lockstep);
+    if (opt.lds_table > 1 ||
+        (opt.lds_table == 1 &&
+         ((u64)rows_of(opt.width) * cols_of(opt.width) * ks->entry_words * 4 > MK_LDS_TABLE_BYTES || opt.blocks != 2 ||
+          opt.lockstep != 0 || opt.pool != 0)))
+        return fail(MK_ERR_OUT_OF_RANGE,
+                    "lds_table 1 needs a table of at most %u bytes (W <= 5, or W = 6 with limbs 10 or 8), 2 waves per "
+                    "SIMD and no lockstep or pool (lds_table %u, W=%u, limbs %u, blocks %u)",
+                    MK_LDS_TABLE_BYTES, opt.lds_table, opt.width, opt.limbs, opt.blocks);
     int count = 0;
     mk_status st = mk_device_count(&count);
     if (st != MK_OK) return st;
@@ -294,7 +303,8 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
     g->grid = g->cus * 4 * opt.blocks;
     // lockstep 1: one workgroup per CU; 2: workgroups of one wave per SIMD; pool: workgroups
     // of `pool` waves sharing an inversion
-    g->lock_waves = opt.pool != 0 ? (MK_WG_POOL | opt.pool)
+    g->lock_waves = opt.lds_table ? (MK_WG_LDST | 8)
+                    : opt.pool != 0 ? (MK_WG_POOL | opt.pool)
                     : opt.lockstep == 1 ? 4 * opt.blocks
                     : opt.lockstep == 2 ? 4
                                         : 0;

@@ -74,3 +74,9 @@ struct mk_key_words {
 #define MK_NOALLOC_WORDS (2u * 8u * MK_NOALLOC_KEYS)
 // Devices whose staging address is kept.
 #define MK_MAX_DEVICES 64
+
+// Largest comb table the LDS-table search kernel holds in LDS (with its 8 waves' bitap
+// masks and SHA mid-states the workgroup stays below the CU's 160 KB): W <= 5 in any
+// layout (W = 5: 51 x 17 entries, 111 KB of 128-byte entries), W = 6 with 96-byte entries
+// (42 x 33 entries, 133 KB).
+#define MK_LDS_TABLE_BYTES (144u * 1024u)

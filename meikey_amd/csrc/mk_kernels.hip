@@ -170,6 +170,7 @@ struct mk_wave_ctx {
     u64* lmask;
     uint4* stage;
     mk_sha_mid* smid;
+    const u32* ltable;  // the comb table in LDS (the LDS-table kernel), else null
 };
 
 // Tail balancing: the last tiles (one per wave of the grid) are cut into U units of K / U
@@ -188,7 +189,9 @@ __device__ __forceinline__ u32 mk_units_per_tile(u32 K, bool serving) {
 // the active requests). Returns false, having searched nothing, when the claim's search
 // has stopped; otherwise the comb multiply calls row_sync() Keff x (rows - 2) times and
 // the batched inversion calls invert(P) once.
-template <class R, class I>
+// With LDST the comb reads its entries from the LDS copy of the table (cx.ltable) instead
+// of staging them from HBM.
+template <bool LDST, class R, class I>
 __device__ __forceinline__ bool mk_search_claim(const mk_search_args& a, mk_wave_ctx& cx, u64 tile, u64 sub,
                                                 u32 Keff, u64 nth, R row_sync, I invert) {
     const u32 lane = cx.lane;
@@ -254,13 +257,16 @@ __device__ __forceinline__ bool mk_search_claim(const mk_search_args& a, mk_wave
     uint4* stage = cx.stage;
     auto mul = [&](const u32 s[8], fe& yn, fe& yd) {
         fe xn, xd;
-        mk_mul_base_frac_staged(a.table, a.w, s, stage, lane, yn, yd, xn, xd, row_sync);
+        if constexpr (LDST)
+            mk_mul_base_frac_near(cx.ltable, a.w, s, yn, yd, xn, xd);
+        else
+            mk_mul_base_frac_staged(a.table, a.w, s, stage, lane, yn, yd, xn, xd, row_sync);
     };
     // During the back-sweep the staging buffers are idle: a DFA of up to
     // MK_LDS_DFA_STATES states is copied into them so its 43 dependent lookups per
     // key hit LDS instead of L2.
     auto pre_sweep = [&] {
-        if (mt.mode == MK_MODE_REGEX && mt.len <= MK_LDS_DFA_STATES) {
+        if (!LDST && mt.mode == MK_MODE_REGEX && mt.len <= MK_LDS_DFA_STATES) {
             u16* ldfa = (u16*)stage;
             const u16* src = a.dfas + dsc.dfa_off;
             for (u32 i = lane; i < mt.len * 65; i += 64) ldfa[i] = src[i];
@@ -316,23 +322,13 @@ struct mk_lane_invert {
 };
 
 // The search kernel: one wave per workgroup, each wave claims its own tiles.
-extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
-    MK_K(mk_search_kernel)(mk_search_args a) {
-    __shared__ u64 lmask[64];
-    __shared__ uint4 stage[2 * MK_STAGE_CHUNKS * 64];
-    __shared__ mk_sha_mid smid;
-    mk_wave_ctx cx;
-    cx.lane = threadIdx.x;
-    cx.lanes = (u64)gridDim.x * 64;
-    cx.gl = (u64)blockIdx.x * 64 + cx.lane;
-    cx.cached = 0xffffffffu;
-    cx.lmask = lmask;
-    cx.stage = stage;
-    cx.smid = &smid;
+// The claim loop of a wave that claims on its own (`waves` = waves of the grid).
+template <bool LDST>
+__device__ __forceinline__ void mk_wave_claims(const mk_search_args& a, mk_wave_ctx& cx, u64 waves) {
     const u32 K = a.K;
     const u32 U = mk_units_per_tile(K, a.serve != nullptr);
     const u64 total_tiles = a.total_tiles;
-    const u64 tail = U > 1 ? (total_tiles < gridDim.x ? total_tiles : gridDim.x) : 0;
+    const u64 tail = U > 1 ? (total_tiles < waves ? total_tiles : waves) : 0;
     const u64 big_tiles = total_tiles - tail;
     for (;;) {
         u64 claim = 0;
@@ -347,8 +343,52 @@ extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per
         if (a.serve != nullptr && a.serve->n_active == 0) break;
         // claim number: whole tiles first, then the units of the tail
         const u64 nth = tile < big_tiles ? tile : big_tiles + (tile - big_tiles) * U + sub;
-        mk_search_claim(a, cx, tile, sub, Keff, nth, [] {}, mk_lane_invert());
+        mk_search_claim<LDST>(a, cx, tile, sub, Keff, nth, [] {}, mk_lane_invert());
     }
+}
+
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
+    MK_K(mk_search_kernel)(mk_search_args a) {
+    __shared__ u64 lmask[64];
+    __shared__ uint4 stage[2 * MK_STAGE_CHUNKS * 64];
+    __shared__ mk_sha_mid smid;
+    mk_wave_ctx cx;
+    cx.lane = threadIdx.x;
+    cx.lanes = (u64)gridDim.x * 64;
+    cx.gl = (u64)blockIdx.x * 64 + cx.lane;
+    cx.cached = 0xffffffffu;
+    cx.lmask = lmask;
+    cx.stage = stage;
+    cx.smid = &smid;
+    cx.ltable = nullptr;
+    mk_wave_claims<false>(a, cx, gridDim.x);
+}
+
+// The LDS-table search kernel: one workgroup of 8 waves per CU copies the whole comb table
+// (at most MK_LDS_TABLE_BYTES: W <= 5, or W = 6 with 96-byte entries) into LDS once, then
+// every wave claims its own tiles as in mk_search_kernel, its comb reading entries from
+// LDS instead of gathering them from HBM / L2. The price is the field work of the small
+// width's rows.
+extern "C" __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
+    MK_K(mk_search_ldst_kernel)(mk_search_args a) {
+    __shared__ uint4 ltab[MK_LDS_TABLE_BYTES / 16];
+    __shared__ u64 lmask[8][64];
+    __shared__ mk_sha_mid smid[8];
+    const u32 n16 = mk_rows(a.w) * mk_cols(a.w) * (MK_ENTRY_WORDS / 4);  // checked on the host
+    const uint4* src = (const uint4*)a.table;
+    for (u32 i = threadIdx.x; i < n16; i += 512) ltab[i] = src[i];
+    __syncthreads();
+    const u32 wave = threadIdx.x / 64;
+    mk_wave_ctx cx;
+    cx.lane = threadIdx.x % 64;
+    cx.lanes = (u64)gridDim.x * 512;
+    cx.gl = (u64)blockIdx.x * 512 + threadIdx.x;
+    cx.cached = 0xffffffffu;
+    cx.lmask = lmask[wave];
+    cx.stage = nullptr;
+    cx.smid = &smid[wave];
+    cx.ltable = (const u32*)ltab;
+    mk_wave_claims<true>(a, cx, (u64)gridDim.x * 8);
 }
 
 // The inversion shared by the NW waves of a workgroup (the reference's block-pooled
@@ -427,6 +467,7 @@ __device__ __forceinline__ void mk_search_lock(const mk_search_args& a) {
     cx.lmask = lmask[wave];
     cx.stage = stage[wave];
     cx.smid = &smid[wave];
+    cx.ltable = nullptr;
     const u32 K = a.K;
     const u32 U = mk_units_per_tile(K, a.serve != nullptr);
     const u64 total_tiles = a.total_tiles;
@@ -460,7 +501,7 @@ __device__ __forceinline__ void mk_search_lock(const mk_search_args& a) {
             else return fe_invert(p);
         };
         if (c < total_claims && (!whole || c < big_tiles))
-            searched = mk_search_claim(a, cx, tile, sub, Keff, c, row_sync, invert);
+            searched = mk_search_claim<false>(a, cx, tile, sub, Keff, c, row_sync, invert);
         if (!searched) {
             if constexpr (ROWS)
                 for (u32 j = 0; j < Keff * (rows - 2); j++) __builtin_amdgcn_s_barrier();
@@ -580,6 +621,11 @@ static hipError_t mk_k_search(hipStream_t s, u32 grid, u32 lock_waves, const u32
     a.next_index = next_index;
     if (lock_waves == 0) {
         hipLaunchKernelGGL(MK_K(mk_search_kernel), dim3(grid), dim3(64), 0, s, a);
+    } else if (lock_waves == (MK_WG_LDST | 8)) {
+        const u64 table_bytes = (u64)((251 + w - 1) / w) * ((1u << (w - 1)) + 1) * MK_ENTRY_WORDS * 4;
+        if (grid % 8 != 0 || table_bytes > MK_LDS_TABLE_BYTES)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL(MK_K(mk_search_ldst_kernel), dim3(grid / 8), dim3(512), 0, s, a);
     } else {
         // grid waves in workgroups of nw (4 or 8, dividing the grid)
         const u32 nw = lock_waves & ~MK_WG_POOL;

@@ -8,8 +8,11 @@
 #include "../../include/meikey_hip.h"
 #include "mk_kernels.h"
 
-// mk_kernel_set::search lock_waves flag: workgroups of 4 or 8 waves sharing one inversion.
+// mk_kernel_set::search lock_waves flags: workgroups of 4 or 8 waves sharing one inversion
+// (MK_WG_POOL | 4 or 8); the comb table copied into LDS, workgroups of 8 waves
+// (MK_WG_LDST | 8).
 #define MK_WG_POOL 0x100u
+#define MK_WG_LDST 0x200u
 
 struct mk_kernel_set {
     u32 limbs;        // mk_options.limbs code of this layout
@@ -24,7 +27,8 @@ struct mk_kernel_set {
     hipError_t (*derive_seeds)(hipStream_t s, const u32* table, u32 w, const u32* seeds, u32 n, u32* out_pk);
     // The search kernel over `grid` waves: one wave per workgroup (lock_waves = 0), the
     // lockstep variant in workgroups of lock_waves waves (4 or 8; grid a multiple of it),
-    // or, with MK_WG_POOL set, workgroups of 4 or 8 waves that share one inversion.
+    // with MK_WG_POOL set, workgroups of 4 or 8 waves that share one inversion, or with
+    // MK_WG_LDST, 8-wave workgroups reading the comb table from LDS.
     hipError_t (*search)(hipStream_t s, u32 grid, u32 lock_waves, const u32* table, u32 w, u32 K,
                          const mk_search_desc* descs, const mk_seg* segs, u32 nsegs, u64 total_tiles,
                          const u64* masks, const u16* dfas, u32* spill, mk_counters* ctr, mk_cand* cands,

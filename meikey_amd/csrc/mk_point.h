@@ -210,6 +210,27 @@ MK_DEV void mk_mul_base_frac(const u32* __restrict__ table, u32 w, const u32 s[8
     mk_add_niels_frac(acc, mk_entry_niels(next, dn), yn, yd, xn, xd);
 }
 
+// mk_mul_base_frac over a table in LDS: each row's entry is read just before its addition
+// (LDS latency is short), so no entry is held in registers across an addition.
+MK_DEV void mk_mul_base_frac_near(const u32* __restrict__ table, u32 w, const u32 s[8], fe& yn, fe& yd, fe& xn,
+                                  fe& xd) {
+    mk_digits g = mk_digits_of(s, w);
+    const u32 rows = g.rows, cols = mk_cols(w);
+    mk_entry e;
+    i32 d = mk_next_digit(g);
+    mk_load_entry(table, 0, d, cols, e);
+    mk_ext acc = mk_row0_extended(mk_entry_niels(e, d));
+    MK_NO_UNROLL
+    for (u32 row = 1; row + 1 < rows; row++) {
+        d = mk_next_digit(g);
+        mk_load_entry(table, row, d, cols, e);
+        acc = mk_add_niels(acc, mk_entry_niels(e, d));
+    }
+    d = mk_next_digit(g);
+    mk_load_entry(table, rows - 1, d, cols, e);
+    mk_add_niels_frac(acc, mk_entry_niels(e, d), yn, yd, xn, xd);
+}
+
 #if defined(__HIPCC__)
 // Wait for every outstanding memory operation of this wave (the LDS copies included).
 MK_DEV void mk_wait_all() {
