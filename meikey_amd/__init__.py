@@ -250,13 +250,13 @@ def fastest(ordinal: int, candidates):
 
 def fastest_staged(ordinal: int = 0):
     """The staged autotuner: (best Options, [(width, batch, blocks, keys/s, status, limbs,
-    lockstep, pool), ...])."""
+    lockstep, pool, lds_table), ...])."""
     best = Options()
     res = (TuneResult * 64)()
     n = ctypes.c_uint32(0)
     _check(lib().mk_fastest_staged(ordinal, ctypes.byref(best), res, 64, ctypes.byref(n)))
     rows = [(r.options.width, r.options.batch, r.options.blocks, r.keys_per_s, r.status, r.options.limbs,
-             r.options.lockstep, r.options.pool) for r in res[:n.value]]
+             r.options.lockstep, r.options.pool, r.options.lds_table) for r in res[:n.value]]
     return best, rows
 
 

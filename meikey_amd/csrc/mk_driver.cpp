@@ -285,7 +285,8 @@ mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* resul
     }
     mk_options w0 = base;
     mk_status st = stage(c0, w0);
-    // stage 2: comb widths with that layout
+    // stage 2: comb widths with that layout, gathered from HBM / L2, and the W = 5 table
+    // held in LDS
     std::vector<mk_options> c1;
     for (uint32_t w : {16u, 20u, 22u, 24u}) {
         mk_options o = base;
@@ -293,11 +294,19 @@ mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* resul
         o.width = w;
         c1.push_back(o);
     }
+    {
+        mk_options o = base;
+        o.limbs = w0.limbs;
+        o.width = 5;
+        o.blocks = 2;
+        o.lds_table = 1;
+        c1.push_back(o);
+    }
     mk_options w1 = w0;
     if (st == MK_OK) st = stage(c1, w1);
     // stage 3: the inversion at the best width: keys per lane, each lane inverting its own
-    // batch, or one inversion shared by the 8 waves of a workgroup (8 keys per lane, the
-    // reference's pooled shape, and 32)
+    // batch, or one inversion shared by the 4 waves of a workgroup (8 keys per lane, the
+    // reference's pooled shape, and 32; measured on MI355X, 4-wave pools beat 8-wave ones)
     std::vector<mk_options> c2;
     for (uint32_t b : {32u, 64u, 128u, 256u}) {
         if (b == w1.batch) continue;
@@ -306,11 +315,12 @@ mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* resul
         c2.push_back(o);
     }
     for (uint32_t b : {8u, 32u}) {
+        if (w1.lds_table) break;  // the LDS-table kernel inverts per lane
         mk_options o = w1;
         o.batch = b;
         o.blocks = 2;
         o.lockstep = 0;
-        o.pool = 8;
+        o.pool = 4;
         c2.push_back(o);
     }
     mk_options w2 = w1;
@@ -319,13 +329,12 @@ mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* resul
         st = stage(c2, w2);
     }
     // stage 4: resident waves per SIMD, and workgroups that add each comb row in lockstep
-    // (a pooled winner keeps 2 waves per SIMD and tries its 4-wave workgroups instead)
+    // (a pooled winner tries 8-wave pools and one wave per SIMD instead)
     mk_options w3 = w2;
-    if (st == MK_OK) {
+    if (st == MK_OK && !w2.lds_table) {
         mk_options o = w2, l = w2;
         if (w2.pool != 0) {
-            o.pool = 4;
-            l.pool = 4;
+            o.pool = 8;
             l.blocks = 1;
         } else {
             o.blocks = w2.blocks == 2 ? 1 : 2;

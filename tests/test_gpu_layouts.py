@@ -141,13 +141,13 @@ def test_headline_search_at_the_production_width(oracle, limbs):
 def test_staged_autotuner_chooses_the_layout():
     """fastest_staged (gpu.rs:1085-1113): the first stage measures the three layouts
     (gpu.rs:364-370 measures every Limbs first), every later stage keeps the layout it
-    chose, the inversion stage includes the pooled settings, and the final setting is
-    the fastest of the last stage."""
+    chose, the width stage includes an LDS-resident table and the inversion stage the
+    pooled settings, and the final setting is the fastest of the last stage."""
     say("staged autotuner")
     best, rows = M.fastest_staged(0)
     for r in rows:
-        say(f"  W={r[0]} K={r[1]} waves={r[2]} limbs={r[5]} lockstep={r[6]} pool={r[7]}: {r[3] / 1e9:.3f} G keys/s "
-            f"status {r[4]}")
+        say(f"  W={r[0]} K={r[1]} waves={r[2]} limbs={r[5]} lockstep={r[6]} pool={r[7]} lds={r[8]}: "
+            f"{r[3] / 1e9:.3f} G keys/s status {r[4]}")
     stage1 = rows[:3]
     assert sorted(r[5] for r in stage1) == sorted([M.LIMBS_R25U, M.LIMBS_R25, M.LIMBS_R32])
     assert all(r[4] == 0 and r[3] > 1e8 for r in stage1)
@@ -155,6 +155,8 @@ def test_staged_autotuner_chooses_the_layout():
     assert best.limbs == chosen and all(r[5] == chosen for r in rows[3:])
     last = rows[-3:]
     top = max((r for r in last if r[4] == 0), key=lambda r: r[3])
-    assert (best.width, best.batch, best.blocks, best.lockstep, best.pool) == (top[0], top[1], top[2], top[6], top[7])
-    # the inversion stage measures pooled settings beside the per-lane ones
-    assert any(r[7] == 8 and r[1] == 8 for r in rows)
+    assert (best.width, best.batch, best.blocks, best.lockstep, best.pool, best.lds_table) == \
+        (top[0], top[1], top[2], top[6], top[7], top[8])
+    # the width stage measures an LDS-resident table, the inversion stage pooled settings
+    assert any(r[8] == 1 and r[0] == 5 and r[4] == 0 for r in rows)
+    assert any(r[7] == 4 and r[1] == 8 and r[4] == 0 for r in rows)
