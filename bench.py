@@ -131,9 +131,12 @@ def main():
         "metric": METRIC, "value": rate, "unit": "keys/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-        "config": {"workload": "secure-mode --bench search, 2^32 ChaCha20 candidates of one master key "
-                               "(random.Random(2)), 6-char case-sensitive suffix",
-                   "candidates": keys, "needle": needle.decode(), "width": args.width,
+        "config": {"workload": f"secure-mode --bench search, {args.steps} launches of 2^29 consecutive ChaCha20 "
+                               f"candidates ({keys} in all; 8 launches cover the 2^32 range once) of one master key "
+                               f"(random.Random(2)), 6-char case-sensitive suffix",
+                   "candidates": keys,
+                   "inversion": (f"one per {args.batch} keys of a lane, pooled over the {args.pool} waves of a "
+                                 f"workgroup" if args.pool else f"one per {args.batch} keys of a lane"), "needle": needle.decode(), "width": args.width,
                    "keys_per_thread": args.batch, "waves_per_simd": args.blocks, "grid_waves": info.grid_waves,
                    "lockstep": args.lockstep, "pool": args.pool, "limbs": args.limbs,
                    "hits": len(all_hits)},
