@@ -173,13 +173,15 @@ struct mk_wave_ctx {
     const u32* ltable;  // the comb table in LDS (the LDS-table kernel), else null
 };
 
-// Tail balancing: the last tiles (one per wave of the grid) are cut into U units of K / U
-// keys per lane, claimed one by one from a second counter, so the waves finish within a
-// fraction of a tile of each other. Units keep at least 64 keys per lane (the inversion
-// is shared by a unit's keys). A serving step of one tile per wave is therefore claimed
-// entirely in units; there they are 128 keys, which balance the step's mix of matchers
-// at half the inversions of 64-key units (0.5% more served keys/s; whole tiles serve 5%
-// fewer).
+// Tail balancing: the last tiles (two per wave of the grid for a wave that claims on its
+// own, one per wave for workgroup claims) are cut into U units of K / U keys per lane,
+// claimed one by one from a second counter, so the waves finish within a fraction of a
+// tile of each other. Units keep at least 64 keys per lane (the inversion is shared by a
+// unit's keys); two tiles per wave in units measured 0.5% faster than one, four and
+// eight slower again, and 32-key units slower than 64. A serving step of one tile per
+// wave is therefore claimed entirely in units; there they are 128 keys, which balance
+// the step's mix of matchers at half the inversions of 64-key units (0.5% more served
+// keys/s; whole tiles serve 5% fewer).
 __device__ __forceinline__ u32 mk_units_per_tile(u32 K, bool serving) {
     return serving ? ((K >= 128 && K % 128 == 0) ? K / 128 : 1) : ((K >= 128 && K % 64 == 0) ? K / 64 : 1);
 }
@@ -328,7 +330,7 @@ __device__ __forceinline__ void mk_wave_claims(const mk_search_args& a, mk_wave_
     const u32 K = a.K;
     const u32 U = mk_units_per_tile(K, a.serve != nullptr);
     const u64 total_tiles = a.total_tiles;
-    const u64 tail = U > 1 ? (total_tiles < waves ? total_tiles : waves) : 0;
+    const u64 tail = U > 1 ? (total_tiles < 2 * waves ? total_tiles : 2 * waves) : 0;
     const u64 big_tiles = total_tiles - tail;
     for (;;) {
         u64 claim = 0;
