@@ -414,12 +414,14 @@ def reference_cpu_leg(key, needle, cores):
 CPU_SHARE = 16
 
 
-def simd_cpu_leg(gpu, key, needle, seconds):
+def simd_cpu_leg(gpu, key, needle, seconds, low=None, side_seconds=5.0):
     """The restatement of the reference's SIMD CPU search path (oracle/libmeikey_simd.so:
     AVX-512 IFMA 5 x 51-bit lanes, or AVX2 where the CPU has no IFMA; 8-way SHA-512; W = 12
     comb; 256-key batch inversion; unsigned-first matching), timed on pinned threads over
-    the headline workload. Its public keys of 2^16 sampled candidates are compared with the
-    GPU's first. None where the CPU has neither instruction set."""
+    the headline workload (`seconds`), and for `side_seconds` each over the low-security
+    workload (counter seeds, 8-char case-insensitive substring) and the 64 mixed searches
+    of the batched workload, one after another. Its public keys of 2^16 sampled candidates
+    are compared with the GPU's first. None where the CPU has neither instruction set."""
     from oracle import simd as S
     try:
         S.lib()
@@ -439,14 +441,36 @@ def simd_cpu_leg(gpu, key, needle, seconds):
         checked += n
         wrong += sum(1 for i in range(n) if a[32 * i:32 * i + 32] != b[32 * i:32 * i + 32])
     r = S.bench(key, needle, True, 1, seconds, threads=threads)
-    return {"value": r["keys_per_s"], "unit": "keys/s", "cores": threads, "kind": "port",
-            "sample": f"{r['keys']} consecutive candidates of the headline key and suffix in {r['seconds']:.2f} s "
-                      f"on {threads} pinned threads, each from its own 2^40-aligned start (restatement of the "
-                      f"reference's SIMD CPU search, {isa} engine, W = 12 comb, 256-key inversion, unsigned-first "
-                      f"matching)",
-            "cpu_model": model, "isa": isa, "affinity_cpus": affinity,
-            "per_thread": r["keys_per_s"] / threads, "hits": r["hits"],
-            "simd_keys_checked_against_gpu": checked, "simd_keys_differing_from_gpu": wrong}
+    out = {"value": r["keys_per_s"], "unit": "keys/s", "cores": threads, "kind": "port",
+           "sample": f"{r['keys']} consecutive candidates of the headline key and suffix in {r['seconds']:.2f} s "
+                     f"on {threads} pinned threads, each from its own 2^40-aligned start (restatement of the "
+                     f"reference's SIMD CPU search, {isa} engine, W = 12 comb, 256-key inversion, unsigned-first "
+                     f"matching)",
+           "cpu_model": model, "isa": isa, "affinity_cpus": affinity,
+           "per_thread": r["keys_per_s"] / threads, "hits": r["hits"],
+           "simd_keys_checked_against_gpu": checked, "simd_keys_differing_from_gpu": wrong}
+    if side_seconds > 0:
+        import meikey_amd as M
+        from meikey_amd import workloads as W
+        if low is not None:
+            lkey, lneedle = low
+            lr = S.bench(lkey, lneedle, False, M.MODE_CONTAINS, side_seconds, threads=threads, source=M.SEED_COUNTER)
+            out["low_security"] = {"keys_per_s": lr["keys_per_s"], "keys": lr["keys"], "seconds": lr["seconds"],
+                                   "hits": lr["hits"], "per_thread": lr["keys_per_s"] / threads}
+        specs, _, _ = W.batched(1 << 26)
+        keys = hits = 0
+        t = 0.0
+        for sp in specs:
+            br = S.bench(sp["key"], sp["needle"], not sp["fold"], sp["mode"], side_seconds / len(specs),
+                         threads=threads)
+            keys += br["keys"]
+            hits += br["hits"]
+            t += br["seconds"]
+        out["batched"] = {"keys_per_s": keys / t, "keys": keys, "seconds": t, "hits": hits, "searches": len(specs),
+                          "per_thread": keys / t / threads,
+                          "sample": f"each of the 64 searches for {side_seconds / len(specs):.3f} s on all threads, "
+                                    f"one after another"}
+    return out
 
 
 def cpu_baseline(gpu, key, needle, served=(), low=None, simd_seconds=10.0):
@@ -495,7 +519,7 @@ def cpu_baseline(gpu, key, needle, served=(), low=None, simd_seconds=10.0):
               "checked_keys": chunks * per, "mismatched_chunks": mismatches}
     # the SIMD restatement of the reference's CPU path is the baseline; the reference
     # kernel's host build and the scalar oracle are reported beside it
-    simd = simd_cpu_leg(gpu, key, needle, simd_seconds)
+    simd = simd_cpu_leg(gpu, key, needle, simd_seconds, low)
     ref = reference_cpu_leg(key, needle, cores)
     out = dict(simd) if simd else dict(ref) if ref else dict(port)
     if simd and ref:
