@@ -56,8 +56,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--blocks", type=int, default=2)
     ap.add_argument("--limbs", type=int, default=None,
-                    help="field / table layout: 10 (packed ten-limb), 11 (ten-limb entries), 8 (eight words); "
-                         "default: the library's")
+                    help="field / table layout: 10 (packed ten-limb), 11 (ten-limb entries), 8 (eight words), "
+                         "26 / 27 (Karatsuba ten-limb, packed / limb entries); default: the library's")
     ap.add_argument("--pool", type=int, default=None,
                     help="4 or 8: workgroups of that many waves share one inversion (default: the library's)")
     ap.add_argument("--lockstep", type=int, default=None,

@@ -53,18 +53,21 @@ typedef enum {
  *   MK_LIMBS_R25  "10"  ten 25.5-bit limbs, table entries packed as 3 x 32 bytes (96 B)
  *   MK_LIMBS_R25U "10u" ten 25.5-bit limbs, table entries stored as limbs (128 B)
  *   MK_LIMBS_R32  "8"   eight 32-bit words (full 2^256 range, carries folded by 38), packed entries
- * The Karatsuba layouts "10k" / "10ku" and the floating-point "5f" are not built
- * (MK_ERR_OUT_OF_RANGE). */
+ *   MK_LIMBS_K26  "10k"  ten 26-bit limbs, Karatsuba products (75 limb products), packed entries
+ *   MK_LIMBS_K26U "10ku" ten 26-bit limbs, Karatsuba products, table entries stored as limbs (128 B)
+ * The floating-point "5f" is not built: the path is integer only (MK_ERR_OUT_OF_RANGE). */
 #define MK_LIMBS_R25 10
 #define MK_LIMBS_R25U 11
 #define MK_LIMBS_R32 8
+#define MK_LIMBS_K26 26
+#define MK_LIMBS_K26U 27
 
 /* Kernel settings: --gpu-width / --gpu-batch / --gpu-limbs (src/controller/cli.rs:85-111,
  * gpu.rs:185-240 GpuOptions). */
 typedef struct {
     uint32_t width;  /* comb table width W, 2..28 (reference 2..20); table = rows(W) x (2^(W-1)+1) x 128 B */
     uint32_t batch;  /* keys per thread per tile, 1..256 (one inversion per `batch` keys per lane) */
-    uint32_t limbs;  /* MK_LIMBS_R25 / MK_LIMBS_R25U / MK_LIMBS_R32 */
+    uint32_t limbs;  /* MK_LIMBS_R25 / MK_LIMBS_R25U / MK_LIMBS_R32 / MK_LIMBS_K26 / MK_LIMBS_K26U */
     uint32_t blocks; /* resident waves per SIMD the grid is sized for, 1..8 (reference: blocks per SM) */
     uint32_t lockstep; /* 0: one wave per workgroup; 1: the waves of a CU form one workgroup and add each
                           comb row together, which keeps the CU's address translation on one or two rows
@@ -74,7 +77,7 @@ typedef struct {
                           the waves; the reference's block-pooled inversion, kernel.cu:1529-1541).
                           8 needs blocks 2, 4 needs blocks 1..2; not with lockstep */
     uint32_t lds_table; /* 1: each CU copies the whole comb table into LDS and its 8 waves read it there
-                           (tables up to 144 KB: W <= 5, or W = 6 with limbs 10 or 8); blocks 2, no
+                           (tables up to 144 KB: W <= 5, or W = 6 with limbs 10, 8 or 26); blocks 2, no
                            lockstep or pool. 0: entries gathered from HBM / L2 */
 } mk_options;
 
@@ -160,10 +163,10 @@ mk_status mk_derive_seeds(mk_gpu* g, const uint8_t* seeds, uint32_t n, uint8_t* 
 
 /* Raw comb-table entries (row, k) as stored on the device, for tests of the table build.
  * Each holds the reference's (y+x, y-x, 2dxy) halved: (y+x)/2, (y-x)/2, dxy (row 0: xy), every
- * field canonical. MK_LIMBS_R25U: 32 words, three 10-limb fields, limbs 0-7 of each in words
- * 0-7 / 8-15 / 16-23, limbs 8-9 in words 24-25 / 26-27 / 28-29, words 30-31 zero. Packed
- * layouts (MK_LIMBS_R25, MK_LIMBS_R32): 24 words, field f as a little-endian 256-bit value in
- * words 8f .. 8f+7. */
+ * field canonical. MK_LIMBS_R25U and MK_LIMBS_K26U: 32 words, three 10-limb fields (25.5-bit
+ * or 26-bit limbs), limbs 0-7 of each in words 0-7 / 8-15 / 16-23, limbs 8-9 in words 24-25 /
+ * 26-27 / 28-29, words 30-31 zero. Packed layouts (MK_LIMBS_R25, MK_LIMBS_R32, MK_LIMBS_K26):
+ * 24 words, field f as a little-endian 256-bit value in words 8f .. 8f+7. */
 mk_status mk_table_read(mk_gpu* g, uint32_t row, uint32_t k, uint32_t count, uint32_t* words);
 
 /* ------------------------------------------------------------------------------------

@@ -23,7 +23,11 @@ MODE_REGEX = 2
 LIMBS_R25 = 10   # "10": ten 25.5-bit limbs, packed 96-byte table entries
 LIMBS_R25U = 11  # "10u": ten 25.5-bit limbs, 128-byte entries of limbs
 LIMBS_R32 = 8    # "8": eight 32-bit words, packed entries
-ENTRY_WORDS = {LIMBS_R25: 24, LIMBS_R25U: 32, LIMBS_R32: 24}
+LIMBS_K26 = 26   # "10k": ten 26-bit limbs with Karatsuba products, packed entries
+LIMBS_K26U = 27  # "10ku": ten 26-bit limbs with Karatsuba products, 128-byte entries of limbs
+ENTRY_WORDS = {LIMBS_R25: 24, LIMBS_R25U: 32, LIMBS_R32: 24, LIMBS_K26: 24, LIMBS_K26U: 32}
+# --gpu-limbs codes (src/controller/cli.rs:105 of the reference) to mk_options.limbs
+LIMBS_BY_CODE = {"10": LIMBS_R25, "10u": LIMBS_R25U, "8": LIMBS_R32, "10k": LIMBS_K26, "10ku": LIMBS_K26U}
 
 STATUS = {
     0: "ok",

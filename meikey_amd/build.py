@@ -27,10 +27,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-mllvm", "-amdgpu-sched-strategy=max-ilp"]
-# mk_kernels.hip builds the "10u" layout; mk_kernels_p10.hip and mk_kernels_r32.hip include
-# it for the packed ten-limb and the eight-word layouts.
+# mk_kernels.hip builds the "10u" layout; mk_kernels_p10.hip, mk_kernels_r32.hip,
+# mk_kernels_k26.hip and mk_kernels_k26u.hip include it for the packed ten-limb, the
+# eight-word and the two Karatsuba layouts.
 HIP_SOURCES = [os.path.join(CSRC, f) for f in ("mk_kernels.hip", "mk_kernels_p10.hip", "mk_kernels_r32.hip",
-                                                 "mk_host.cpp", "mk_pattern.cpp", "mk_serve.cpp", "mk_driver.cpp")]
+                                                 "mk_kernels_k26.hip", "mk_kernels_k26u.hip", "mk_host.cpp",
+                                                 "mk_pattern.cpp", "mk_serve.cpp", "mk_driver.cpp")]
 
 
 def _run(cmd: list[str]) -> None:
@@ -77,7 +79,8 @@ def build_hip(out: str = LIB) -> str:
 
 
 # The CPU test builds of the arithmetic headers, one per layout (as the kernels are built).
-HOSTCHECK_LAYOUTS = {"": [], "_p10": ["-DMK_PACKED=1"], "_r32": ["-DMK_FIELD=8", "-DMK_PACKED=1"]}
+HOSTCHECK_LAYOUTS = {"": [], "_p10": ["-DMK_PACKED=1"], "_r32": ["-DMK_FIELD=8", "-DMK_PACKED=1"],
+                     "_k26": ["-DMK_FIELD=26", "-DMK_PACKED=1"], "_k26u": ["-DMK_FIELD=26"]}
 
 
 def hostcheck_path(layout: str = "") -> str:

@@ -51,18 +51,22 @@ MK_DEV u32 mk_cols(u32 w) { return (1u << (w - 1)) + 1; }
 // Field representation and comb-table layout of one build of the kernels (the reference's
 // --gpu-limbs codes, src/service/gpu.rs:70-106). Each is compiled into its own set of
 // kernels; mk_options.limbs picks the set at run time.
-//   MK_FIELD 10: ten 25.5-bit limbs (mk_field.h); 8: eight 32-bit words (mk_field8.h)
+//   MK_FIELD 10: ten 25.5-bit limbs (mk_field.h); 8: eight 32-bit words (mk_field8.h);
+//            26: ten 26-bit limbs with Karatsuba products (mk_field26.h)
 //   MK_PACKED 0: a table entry holds its three fields as limbs: 30 words + 2 pad words,
-//                one 128-byte line ("10u");
+//                one 128-byte line ("10u", "10ku");
 //             1: a table entry holds them as canonical little-endian 256-bit values,
-//                3 x 8 words = 96 bytes ("10", "8")
+//                3 x 8 words = 96 bytes ("10", "8", "10k")
 #ifndef MK_FIELD
 #define MK_FIELD 10
 #endif
 #ifndef MK_PACKED
 #define MK_PACKED 0
 #endif
-#define MK_FE_LIMBS MK_FIELD
+#define MK_FE_LIMBS (MK_FIELD == 8 ? 8 : 10)
+// The mk_options.limbs code of this build (include/meikey_hip.h).
+#define MK_LIMBS_CODE                                                                                    \
+    (MK_FIELD == 8 ? 8u : MK_FIELD == 26 ? (MK_PACKED ? 26u : 27u) : (MK_PACKED ? 10u : 11u))
 // One comb-table entry: the affine point as ((y+x)/2, (y-x)/2, dxy) (row 0: xy).
 #define MK_ENTRY_WORDS (MK_PACKED ? 24 : 32)
 

@@ -277,7 +277,8 @@ mk_status mk_fastest_staged(int ordinal, mk_options* best, mk_tune_result* resul
     // stage 1: the field / table layouts (gpu.rs:364-370 measures every Limbs first) at
     // W = 16 and the default keys per lane and waves per SIMD
     std::vector<mk_options> c0;
-    for (uint32_t limbs : {(uint32_t)MK_LIMBS_R25U, (uint32_t)MK_LIMBS_R25, (uint32_t)MK_LIMBS_R32}) {
+    for (uint32_t limbs : {(uint32_t)MK_LIMBS_R25U, (uint32_t)MK_LIMBS_R25, (uint32_t)MK_LIMBS_R32,
+                           (uint32_t)MK_LIMBS_K26U, (uint32_t)MK_LIMBS_K26}) {
         mk_options o = base;
         o.width = 16;
         o.limbs = limbs;

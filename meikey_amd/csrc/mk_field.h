@@ -22,6 +22,8 @@
 
 #if MK_FIELD == 8
 #include "mk_field8.h"
+#elif MK_FIELD == 26
+#include "mk_field26.h"
 #else
 
 struct fe {

@@ -248,12 +248,17 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
                     "the table width must be %d-%d, the keys per thread 1-%u and the waves per SIMD 1-8 (W=%u, "
                     "batch %u, blocks %u)",
                     MK_MIN_W, MK_MAX_W, kMaxBatch, opt.width, opt.batch, opt.blocks);
-    const mk_kernel_set* ks = opt.limbs == MK_LIMBS_R25U  ? &mk_kernels_u10
-                              : opt.limbs == MK_LIMBS_R25 ? &mk_kernels_p10
-                              : opt.limbs == MK_LIMBS_R32 ? &mk_kernels_r32
-                                                          : nullptr;
+    const mk_kernel_set* ks = opt.limbs == MK_LIMBS_R25U   ? &mk_kernels_u10
+                              : opt.limbs == MK_LIMBS_R25  ? &mk_kernels_p10
+                              : opt.limbs == MK_LIMBS_R32  ? &mk_kernels_r32
+                              : opt.limbs == MK_LIMBS_K26  ? &mk_kernels_k26
+                              : opt.limbs == MK_LIMBS_K26U ? &mk_kernels_k26u
+                                                           : nullptr;
     if (!ks)
-        return fail(MK_ERR_OUT_OF_RANGE, "limb layout %u is not available (10, 11 = \"10u\" or 8)", opt.limbs);
+        return fail(MK_ERR_OUT_OF_RANGE,
+                    "limb layout %u is not available (10, 11 = \"10u\", 8, 26 = \"10k\" or 27 = \"10ku\"; the "
+                    "floating-point \"5f\" is not built)",
+                    opt.limbs);
     if (opt.lockstep > 2 || (opt.lockstep == 1 && opt.blocks > 2))
         return fail(MK_ERR_OUT_OF_RANGE,
                     "lockstep is 0, 1 (one workgroup per CU, 1 or 2 waves per SIMD) or 2 (workgroups of one wave per "
@@ -269,7 +274,7 @@ mk_status mk_open(int ordinal, const mk_options* opt_in, mk_gpu** out) {
          ((u64)rows_of(opt.width) * cols_of(opt.width) * ks->entry_words * 4 > MK_LDS_TABLE_BYTES || opt.blocks != 2 ||
           opt.lockstep != 0 || opt.pool != 0)))
         return fail(MK_ERR_OUT_OF_RANGE,
-                    "lds_table 1 needs a table of at most %u bytes (W <= 5, or W = 6 with limbs 10 or 8), 2 waves per "
+                    "lds_table 1 needs a table of at most %u bytes (W <= 5, or W = 6 with packed entries: limbs 10, 8 or 26), 2 waves per "
                     "SIMD and no lockstep or pool (lds_table %u, W=%u, limbs %u, blocks %u)",
                     MK_LDS_TABLE_BYTES, opt.lds_table, opt.width, opt.limbs, opt.blocks);
     int count = 0;

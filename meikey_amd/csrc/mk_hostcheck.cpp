@@ -39,7 +39,8 @@ void hc_secret_scalar_counter(const u32 key[8], u64 index, u32 s[8]) {
     mk_secret_scalar_mid(mid, mk_be_word((u32)t, (u32)(t >> 32)), s);
 }
 
-// The build's field and table layout: 10 * MK_FIELD + MK_PACKED (100 "10u", 101 "10", 81 "8").
+// The build's field and table layout: 10 * MK_FIELD + MK_PACKED (100 "10u", 101 "10", 81 "8",
+// 261 "10k", 260 "10ku").
 int hc_layout(void) { return 10 * MK_FIELD + MK_PACKED; }
 
 #define FE_BYTES (4 * MK_FE_LIMBS)

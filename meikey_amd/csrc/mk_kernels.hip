@@ -21,8 +21,9 @@
 //
 // This file is compiled once per field representation and table layout (mk_common.h):
 // as it stands for ten 25.5-bit limbs and 128-byte entries of limbs ("10u"), and through
-// mk_kernels_p10.hip (packed 96-byte entries, "10") and mk_kernels_r32.hip (eight 32-bit
-// words, "8"), which set MK_FIELD / MK_PACKED / MK_VARIANT first. Each build lives in its
+// mk_kernels_p10.hip (packed 96-byte entries, "10"), mk_kernels_r32.hip (eight 32-bit
+// words, "8"), mk_kernels_k26.hip and mk_kernels_k26u.hip (ten 26-bit limbs with Karatsuba
+// products, packed "10k" and limbs "10ku"), which set MK_FIELD / MK_PACKED / MK_VARIANT first. Each build lives in its
 // own namespace with kernel names suffixed by MK_VARIANT, and exports one mk_kernel_set.
 #include <hip/hip_runtime.h>
 
@@ -658,7 +659,7 @@ static hipError_t mk_k_confirm(hipStream_t s, const u32* table, u32 w, const mk_
 #if !defined(__HIP_DEVICE_COMPILE__)  // host code only: the device pass has no launchers
 extern const mk_kernel_set MK_K(mk_kernels);
 const mk_kernel_set MK_K(mk_kernels) = {
-    MK_PACKED ? (MK_FIELD == 8 ? MK_LIMBS_R32 : MK_LIMBS_R25) : MK_LIMBS_R25U,
+    MK_LIMBS_CODE,
     MK_ENTRY_WORDS,
     (u32)sizeof(MK_NS::mk_ext),
     MK_NS::mk_k_table_bases,

@@ -39,10 +39,13 @@ struct mk_kernel_set {
 };
 
 // "10u": ten 25.5-bit limbs, 128-byte entries of limbs; "10": the same arithmetic with
-// packed 96-byte entries; "8": eight 32-bit words, packed entries.
+// packed 96-byte entries; "8": eight 32-bit words, packed entries; "10k" / "10ku": ten
+// 26-bit limbs with Karatsuba products, packed entries / entries of limbs.
 extern const mk_kernel_set mk_kernels_u10;
 extern const mk_kernel_set mk_kernels_p10;
 extern const mk_kernel_set mk_kernels_r32;
+extern const mk_kernel_set mk_kernels_k26;
+extern const mk_kernel_set mk_kernels_k26u;
 
 // Largest extended point of any layout (4 field elements x 10 limbs).
 #define MK_EXT_BYTES (4 * 10 * 4)

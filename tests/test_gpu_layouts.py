@@ -1,8 +1,9 @@
 """The other field / table layouts on the device (mk_options.limbs): ten 25.5-bit limbs
-with packed 96-byte entries ("10") and eight 32-bit words ("8"). Each must give exactly
-the keys, table and hit sets of the oracle and of the "10u" build: known answers, bulk
-derivation, sampled table entries, mixed search batches, a served request, and the
-headline search at the production width."""
+with packed 96-byte entries ("10"), eight 32-bit words ("8") and ten 26-bit limbs with
+Karatsuba products, packed ("10k") or as limbs ("10ku"). Each must give exactly the keys,
+table and hit sets of the oracle and of the "10u" build: known answers, bulk derivation,
+sampled table entries, mixed search batches, a served request, and the headline search at
+the production width."""
 import os
 import random
 import struct
@@ -15,7 +16,8 @@ from tests import textbook as T
 
 pytestmark = pytest.mark.gpu
 THREADS = min(os.cpu_count() or 1, 16)
-LAYOUTS = [M.LIMBS_R25, M.LIMBS_R32]
+LAYOUTS = [M.LIMBS_R25, M.LIMBS_R32, M.LIMBS_K26, M.LIMBS_K26U]
+IDS = ["10", "8", "10k", "10ku"]
 
 
 def say(msg):
@@ -38,7 +40,7 @@ def plain():
     g.close()
 
 
-@pytest.fixture(scope="module", params=LAYOUTS, ids=["10", "8"])
+@pytest.fixture(scope="module", params=LAYOUTS, ids=IDS)
 def gpu_layout(request):
     say(f"open gpu W=16 batch=32 limbs={request.param}")
     g = M.Gpu(0, width=16, batch=32, limbs=request.param)
@@ -61,7 +63,22 @@ def test_known_answers_and_bulk_derivation(gpu_layout, oracle):
         assert gpu_layout.derive(key, source, first, n) == oracle.derive(key, source, first, n, threads=THREADS)
 
 
-def test_packed_table_entries(gpu_layout, oracle):
+def _entry_values(words, limbs):
+    """The three canonical fields of a raw entry: 24 words of packed 256-bit values, or ("10ku")
+    32 words of ten 26-bit limbs per field (limbs 0-7 in words 8f .. 8f+7, 8-9 in 24+2f, 25+2f)."""
+    if limbs != M.LIMBS_K26U:
+        assert len(words) == 24
+        return [int.from_bytes(struct.pack("<8I", *words[8 * f:8 * f + 8]), "little") for f in range(3)]
+    assert len(words) == 32 and words[30] == 0 and words[31] == 0
+    out = []
+    for f in range(3):
+        ls = [words[8 * f + i] if i < 8 else words[24 + 2 * f + i - 8] for i in range(10)]
+        assert all(x < (1 << 26) for x in ls) and ls[9] < (1 << 21)
+        out.append(sum(x << (26 * i) for i, x in enumerate(ls)))
+    return out
+
+
+def test_table_entries(gpu_layout, oracle):
     info = gpu_layout.info()
     rows, cols = info.rows, info.cols
     rng = random.Random(191)
@@ -70,8 +87,8 @@ def test_packed_table_entries(gpu_layout, oracle):
         (rng.randrange(rows), rng.randrange(cols)) for _ in range(20)]
     for row, k in picks:
         words = gpu_layout.table_words(row, k)
-        assert len(words) == 24
-        ypx, ymx, third = (int.from_bytes(struct.pack("<8I", *words[8 * f:8 * f + 8]), "little") for f in range(3))
+        ypx, ymx, third = _entry_values(words, info.options.limbs)
+        assert max(ypx, ymx, third) < T.P
         n = oracle.comb_entry_niels(16, row, k)
         assert ypx == int.from_bytes(n[0:32], "little") * half % T.P, (row, k)
         assert ymx == int.from_bytes(n[32:64], "little") * half % T.P, (row, k)
@@ -116,7 +133,7 @@ def test_served_request(gpu_layout, oracle):
     assert first and first[0] == r.index and r.pk == oracle.derive(key, 0, r.index, 1)
 
 
-@pytest.mark.parametrize("limbs", LAYOUTS, ids=["10", "8"])
+@pytest.mark.parametrize("limbs", LAYOUTS, ids=IDS)
 def test_headline_search_at_the_production_width(oracle, limbs):
     """W = 24, 256 keys per lane: a planted 6-char case-sensitive suffix over a 2^22 window
     of the headline key gives the oracle's full hit set."""
@@ -139,7 +156,7 @@ def test_headline_search_at_the_production_width(oracle, limbs):
 
 
 def test_staged_autotuner_chooses_the_layout():
-    """fastest_staged (gpu.rs:1085-1113): the first stage measures the three layouts
+    """fastest_staged (gpu.rs:1085-1113): the first stage measures the five integer layouts
     (gpu.rs:364-370 measures every Limbs first), every later stage keeps the layout it
     chose, the width stage includes an LDS-resident table and the inversion stage the
     pooled settings, and the final setting is the fastest of the last stage."""
@@ -148,11 +165,12 @@ def test_staged_autotuner_chooses_the_layout():
     for r in rows:
         say(f"  W={r[0]} K={r[1]} waves={r[2]} limbs={r[5]} lockstep={r[6]} pool={r[7]} lds={r[8]}: "
             f"{r[3] / 1e9:.3f} G keys/s status {r[4]}")
-    stage1 = rows[:3]
-    assert sorted(r[5] for r in stage1) == sorted([M.LIMBS_R25U, M.LIMBS_R25, M.LIMBS_R32])
+    stage1 = rows[:5]
+    assert sorted(r[5] for r in stage1) == sorted([M.LIMBS_R25U, M.LIMBS_R25, M.LIMBS_R32, M.LIMBS_K26,
+                                                   M.LIMBS_K26U])
     assert all(r[4] == 0 and r[3] > 1e8 for r in stage1)
     chosen = max(stage1, key=lambda r: r[3])[5]
-    assert best.limbs == chosen and all(r[5] == chosen for r in rows[3:])
+    assert best.limbs == chosen and all(r[5] == chosen for r in rows[5:])
     last = rows[-3:]
     top = max((r for r in last if r[4] == 0), key=lambda r: r[3])
     assert (best.width, best.batch, best.blocks, best.lockstep, best.pool, best.lds_table) == \
