@@ -332,7 +332,7 @@ typedef struct {
 } mk_tune_result;
 
 mk_status mk_fastest(int ordinal, const mk_options* candidates, uint32_t n, mk_tune_result* results, mk_options* best);
-/* Stage 1: the layouts "10u", "10" and "8" at W = 16 (gpu.rs:364-370); stage 2: widths
+/* Stage 1: the layouts "10u", "10", "8", "10ku" and "10k" at W = 16 (gpu.rs:364-370); stage 2: widths
  * 16/20/22/24 gathered from HBM / L2, and W = 5 held in LDS; stage 3: the inversion,
  * 32/64/128/256 keys per lane each inverting its own batch or 8/32 keys per lane with one
  * inversion per 4-wave workgroup (pool 4); stage 4: 1 or 2 waves per SIMD, and lockstep

@@ -492,8 +492,8 @@ class Gpu:
         return a.value, b.value
 
     def table_words(self, row: int, k: int, count: int = 1, as_array: bool = False):
-        """Raw table entries from (row, k) on, 32 words each for LIMBS_R25U, 24 for the packed
-        layouts (a list, or with as_array a numpy array of shape (count, words))."""
+        """Raw table entries from (row, k) on, 32 words each for LIMBS_R25U and LIMBS_K26U, 24 for
+        the packed layouts (a list, or with as_array a numpy array of shape (count, words))."""
         n = ENTRY_WORDS[self.options.limbs]
         buf = (ctypes.c_uint32 * (n * count))()
         _check(lib().mk_table_read(self._h, row, k, count, buf))
