@@ -34,7 +34,7 @@ def test_lds_table_rejects_what_does_not_fit():
 
 
 @pytest.mark.parametrize("width,limbs", [(5, M.LIMBS_R25U), (5, M.LIMBS_R25), (6, M.LIMBS_R25), (6, M.LIMBS_R32),
-                                         (3, M.LIMBS_R25U)])
+                                         (3, M.LIMBS_R25U), (5, M.LIMBS_K26U), (6, M.LIMBS_K26)])
 def test_lds_table_batches_equal_global_and_oracle(oracle, width, limbs):
     say(f"open gpu W={width} limbs={limbs} lds_table=1 and the global-table twin")
     g = M.Gpu(0, width=width, batch=32, blocks=2, limbs=limbs, lds_table=1)

@@ -45,16 +45,19 @@ def test_pool_rejects_bad_settings():
             M.Gpu(0, width=12, batch=8, **kw)
 
 
-@pytest.mark.parametrize("batch,pool,blocks", [(8, 8, 2), (32, 8, 2), (256, 8, 2), (16, 4, 2), (16, 4, 1)])
-def test_pool_batches_equal_per_lane_and_oracle(oracle, batch, pool, blocks):
-    say(f"open gpu W=16 batch={batch} pool={pool} blocks={blocks} and the per-lane twin")
-    g = M.Gpu(0, width=16, batch=batch, blocks=blocks, pool=pool)
-    ref_gpu = M.Gpu(0, width=16, batch=batch, blocks=blocks)
+@pytest.mark.parametrize("batch,pool,blocks,limbs", [(8, 8, 2, M.LIMBS_R25U), (32, 8, 2, M.LIMBS_R25U),
+                                                    (256, 8, 2, M.LIMBS_R25U), (16, 4, 2, M.LIMBS_R25U),
+                                                    (16, 4, 1, M.LIMBS_R25U), (8, 4, 2, M.LIMBS_K26U),
+                                                    (32, 8, 2, M.LIMBS_K26)])
+def test_pool_batches_equal_per_lane_and_oracle(oracle, batch, pool, blocks, limbs):
+    say(f"open gpu W=16 batch={batch} pool={pool} blocks={blocks} limbs={limbs} and the per-lane twin")
+    g = M.Gpu(0, width=16, batch=batch, blocks=blocks, pool=pool, limbs=limbs)
+    ref_gpu = M.Gpu(0, width=16, batch=batch, blocks=blocks, limbs=limbs)
     try:
         info = g.info()
         assert info.options.pool == pool
         assert info.scratch_bytes == ref_gpu.info().scratch_bytes
-        specs = _mixed_specs(random.Random(400 + batch + pool + blocks), 64 * batch)
+        specs = _mixed_specs(random.Random(400 + batch + pool + blocks + limbs), 64 * batch)
         say("pooled batch")
         hits, tries = g.batch_run(specs)
         ref, rtries = ref_gpu.batch_run(specs)
