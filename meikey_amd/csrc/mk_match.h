@@ -38,20 +38,36 @@ MK_DEV u32 mk_tail_value(const u32 pk[8], int pos) {
 }
 
 // Shift-and over tail positions From..43; returns the final state D and, in seen, the OR
-// of all states. Bit k of D at position p is set when needle[0..k] ends at p.
-template <int From>
-MK_DEV u64 mk_bitap_run(const u32 pk[8], const u64* mask, bool sign_unknown, u64& seen) {
-    u64 D = 0;
+// of all states. Bit k of D at position p is set when needle[0..k] ends at p. S is the
+// state word: u32 for needles of at most 32 characters (the low half of each mask entry,
+// one 32-bit LDS read and 32-bit shift-and per position), u64 for longer ones.
+template <int From, class S>
+MK_DEV S mk_bitap_run(const u32 pk[8], const u64* mask, bool sign_unknown, S& seen) {
+    S D = 0;
     seen = 0;
     MK_UNROLL
     for (int pos = From; pos < MK_TAIL_B64; pos++) {
         const u32 v = mk_tail_value(pk, pos);
-        u64 m = mask[v];
-        if (pos == MK_SIGN_CHAR && sign_unknown) m |= mask[v | 2u];
-        D = ((D << 1) | 1ull) & m;
+        S m = (S)mask[v];
+        if (pos == MK_SIGN_CHAR && sign_unknown) m |= (S)mask[v | 2u];
+        D = ((D << 1) | (S)1) & m;
         seen |= D;
     }
     return D;
+}
+
+template <class S>
+MK_DEV bool mk_bitap_match_w(const u32 pk[8], const u64* mask, u32 len, bool suffix, bool sign_unknown) {
+    const S last = (S)1 << (len - 1);
+    S seen = 0;
+    if (suffix) {
+        const S D = len <= 8    ? mk_bitap_run<MK_TAIL_B64 - 8>(pk, mask, sign_unknown, seen)
+                    : len <= 16 ? mk_bitap_run<MK_TAIL_B64 - 16>(pk, mask, sign_unknown, seen)
+                                : mk_bitap_run<MK_SEARCH_FROM>(pk, mask, sign_unknown, seen);
+        return (D & last) != 0;
+    }
+    mk_bitap_run<MK_SEARCH_FROM>(pk, mask, sign_unknown, seen);
+    return (seen & last) != 0;
 }
 
 // Substring (suffix = false) or suffix match by shift-and over tail positions 1..43.
@@ -59,16 +75,8 @@ MK_DEV u64 mk_bitap_run(const u32 pk[8], const u64* mask, bool sign_unknown, u64
 // position 44 - len, so the automaton starts at 36 for len <= 8 and at 28 for len <= 16
 // (the final state is the same as from position 1).
 MK_DEV bool mk_bitap_match(const u32 pk[8], const u64* mask, u32 len, bool suffix, bool sign_unknown) {
-    const u64 last = 1ull << (len - 1);
-    u64 seen = 0;
-    if (suffix) {
-        const u64 D = len <= 8    ? mk_bitap_run<MK_TAIL_B64 - 8>(pk, mask, sign_unknown, seen)
-                      : len <= 16 ? mk_bitap_run<MK_TAIL_B64 - 16>(pk, mask, sign_unknown, seen)
-                                  : mk_bitap_run<MK_SEARCH_FROM>(pk, mask, sign_unknown, seen);
-        return (D & last) != 0;
-    }
-    mk_bitap_run<MK_SEARCH_FROM>(pk, mask, sign_unknown, seen);
-    return (seen & last) != 0;
+    return len <= 32 ? mk_bitap_match_w<u32>(pk, mask, len, suffix, sign_unknown)
+                     : mk_bitap_match_w<u64>(pk, mask, len, suffix, sign_unknown);
 }
 
 MK_DEV u32 mk_dfa_next(const u16* dfa, u32 s, u32 col) { return dfa[s * 65u + col]; }
