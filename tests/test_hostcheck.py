@@ -206,8 +206,9 @@ def test_bitap_matcher_vs_oracle(oracle, hostcheck):
     for _ in range(6000):
         pk = rng.randbytes(32)
         b64 = T.public_b64(pk)
-        # lengths on both sides of the suffix automaton's start points (8 / 16)
-        L = rng.choice([1, 2, 3, 5, 7, 8, 9, 12, 16, 17, 30, 42, 43])
+        # lengths on both sides of the suffix automaton's start points (8 / 16) and of the
+        # switch from 32-bit to 64-bit automaton states (32)
+        L = rng.choice([1, 2, 3, 5, 7, 8, 9, 12, 16, 17, 30, 31, 32, 33, 42, 43])
         r = rng.random()
         if r < 0.4:
             needle = b64[68 - L:]  # a true suffix of the key
