@@ -90,3 +90,20 @@ def test_no_device_fails_loudly():
     assert e.value.status == 2 and "no HIP device" in str(e.value)
     with pytest.raises(meikey_amd.MeikeyError):
         meikey_amd.fastest_staged(0)
+
+
+def test_constants_match_the_header():
+    """Every numeric #define of include/meikey_hip.h that the Python layer mirrors (seed
+    sources, match modes, the five layout codes) has the header's value, and the
+    --gpu-limbs codes map onto the layouts one to one (the floating-point "5f" has no code)."""
+    text = open(os.path.join(ROOT, "include", "meikey_hip.h")).read()
+    defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"^#define MK_(\w+) (\d+)u?\b", text, flags=re.M)}
+    mirrored = [k for k in defs if hasattr(meikey_amd, k)]
+    assert sorted(mirrored) == sorted(["SEED_CHACHA20", "SEED_COUNTER", "MODE_CONTAINS", "MODE_SUFFIX", "MODE_REGEX",
+                                       "LIMBS_R25", "LIMBS_R25U", "LIMBS_R32", "LIMBS_K26", "LIMBS_K26U"])
+    for k in mirrored:
+        assert getattr(meikey_amd, k) == defs[k], k
+    layouts = {v for k, v in defs.items() if k.startswith("LIMBS_")}
+    assert set(meikey_amd.LIMBS_BY_CODE.values()) == layouts and len(meikey_amd.LIMBS_BY_CODE) == len(layouts)
+    assert sorted(meikey_amd.LIMBS_BY_CODE) == ["10", "10k", "10ku", "10u", "8"]
+    assert set(meikey_amd.ENTRY_WORDS) == layouts
