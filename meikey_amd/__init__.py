@@ -502,20 +502,32 @@ class Gpu:
             return np.frombuffer(buf, dtype=np.uint32).reshape(count, n).copy()
         return list(buf)
 
-    def batch_run(self, specs: Sequence[dict], cap: int = 1 << 20, as_array: bool = False):
+    def batch_run(self, specs: Sequence[dict], cap: int = 1 << 20, as_array: bool = False, out=None):
         """Run many searches in one launch. Each spec: key, seed_source, mode, fold, needle,
         first, count, stop_on_hit (optional). Returns (hits, tries per search); with
-        as_array the hits are one numpy structured array (fields of hit_dtype())."""
+        as_array the hits are one numpy structured array (fields of hit_dtype()). With out, a
+        C-contiguous numpy array of hit_dtype() that the caller keeps between calls, the
+        records are written straight into it and the hits returned are the view out[:n]: no
+        copy, valid until out is reused. More hits than len(out) is MK_ERR_OVERFLOW, as for
+        cap."""
         n = len(specs)
         arr = (SearchSpec * max(n, 1))()
         keep = []
         for i, s in enumerate(specs):
             _fill_spec(arr[i], s, keep)
-        recs = self._hit_buffer(cap)
+        if out is not None:
+            if out.dtype != hit_dtype() or not out.flags["C_CONTIGUOUS"] or not out.flags["WRITEABLE"]:
+                raise ValueError("out must be a writeable C-contiguous array of hit_dtype()")
+            cap = len(out)
+            recs = ctypes.cast(out.ctypes.data, ctypes.POINTER(HitRec)) if cap else None
+        else:
+            recs = self._hit_buffer(cap)
         nh = ctypes.c_uint32(0)
         tries = (ctypes.c_uint64 * max(n, 1))()
         _check(lib().mk_batch_run(self._h, arr, n, recs, cap, ctypes.byref(nh), tries))
         got = min(nh.value, cap)
+        if out is not None:
+            return out[:got], list(tries)[:n]
         return (_hits_array(recs, got) if as_array else _hits(recs, got)), list(tries)[:n]
 
     def close(self) -> None:

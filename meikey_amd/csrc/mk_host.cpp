@@ -119,11 +119,15 @@ static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "hit words are copied a
 void to_records(mk_gpu* g, const mk_hit* hs, u32 n, mk_hit_rec* recs, u32 cap) {
     const u32* order = mk_sort_hits(hs, n, g->sort);
     const u32 m = std::min(n, cap);
-    for (u32 i = 0; i < m; i++) {
-        if (i + 8 < m) __builtin_prefetch(&hs[order[i + 8]]);
-        memcpy(&recs[i], &hs[order[i]], sizeof(mk_hit_rec));
-        recs[i].pad = 0;
-    }
+    const u32 T = mk_host_threads(m);
+    mk_host_parallel(T, [&](u32 t) {
+        const u32 b = (u32)((u64)m * t / T), e = (u32)((u64)m * (t + 1) / T);
+        for (u32 i = b; i < e; i++) {
+            if (i + 8 < e) __builtin_prefetch(&hs[order[i + 8]]);
+            memcpy(&recs[i], &hs[order[i]], sizeof(mk_hit_rec));
+            recs[i].pad = 0;
+        }
+    });
 }
 
 // RFC 8032 s7.1 known answers (secret seed -> public key): TEST 1, 2, 3, 1024, SHA(abc).

@@ -258,6 +258,28 @@ def test_batched_equals_single(gpu, oracle):
         assert [h.index for h in single] == want
 
 
+def test_batch_run_into_caller_buffer(gpu):
+    """batch_run(out=buf) writes the records straight into the caller's array and returns
+    the view buf[:n]: the same records as the copying as_array form. A buffer shorter than
+    the hit list is an overflow, as a short cap is for the C entry point."""
+    import numpy as np
+    rng = random.Random(47)
+    specs = [_spec(rng.randbytes(32), i % 2, b"ab", False, M.MODE_CONTAINS, rng.getrandbits(40), 64 * 32 * 20)
+             for i in range(3)]
+    want, tries = gpu.batch_run(specs, as_array=True)
+    assert len(want) > 40
+    buf = np.zeros(len(want) + 7, dtype=M.hit_dtype())
+    got, t2 = gpu.batch_run(specs, out=buf)
+    assert t2 == tries
+    assert np.shares_memory(got, buf) and len(got) == len(want)
+    assert got.tobytes() == want.tobytes()
+    with pytest.raises(M.MeikeyError) as e:
+        gpu.batch_run(specs, out=np.zeros(40, dtype=M.hit_dtype()))
+    assert e.value.status == 7  # MK_ERR_OVERFLOW
+    with pytest.raises(ValueError):
+        gpu.batch_run(specs, out=np.zeros(8, dtype=np.uint64))
+
+
 def test_config1_full_hit_set(gpu, oracle):
     """Config 1: 2^24 ChaCha20 candidates of the master key from PRNG seed 1, a seeded
     4-char case-insensitive substring; every hit index and key equal to the oracle's."""
