@@ -326,6 +326,9 @@ struct mk_lane_invert {
 
 // The search kernel: one wave per workgroup, each wave claims its own tiles.
 // The claim loop of a wave that claims on its own (`waves` = waves of the grid).
+#ifndef MK_TAIL_SPLIT
+#define MK_TAIL_SPLIT 1
+#endif
 template <bool LDST>
 __device__ __forceinline__ void mk_wave_claims(const mk_search_args& a, mk_wave_ctx& cx, u64 waves) {
     const u32 K = a.K;
@@ -333,6 +336,12 @@ __device__ __forceinline__ void mk_wave_claims(const mk_search_args& a, mk_wave_
     const u64 total_tiles = a.total_tiles;
     const u64 tail = U > 1 ? (total_tiles < 2 * waves ? total_tiles : 2 * waves) : 0;
     const u64 big_tiles = total_tiles - tail;
+    // The last `split` units of the tail (one per wave of the grid, the final round of
+    // claims) are cut once more into halves of K / (2U) keys per lane. Measured on one box
+    // each: 0.3-0.6% faster than without (three and two A/B pairs); halving the last two
+    // rounds, or quartering the last round's halves, no faster again.
+    const u64 units = tail * U;
+    const u64 split = (MK_TAIL_SPLIT && a.serve == nullptr && U > 1 && units >= 2 * waves) ? waves : 0;
     for (;;) {
         u64 claim = 0;
         if (cx.lane == 0) {
@@ -340,8 +349,17 @@ __device__ __forceinline__ void mk_wave_claims(const mk_search_args& a, mk_wave_
             claim = t < big_tiles ? t * U : big_tiles * U + atomicAdd((unsigned long long*)&a.ctr->queue2, 1ull);
         }
         claim = mk_uniform64(claim);
-        const u64 tile = claim / U, sub = claim % U;
-        const u32 Keff = tile >= big_tiles ? K / U : K;
+        u64 tile = claim / U, sub = claim % U;
+        u32 Keff = tile >= big_tiles ? K / U : K;
+        if (split != 0 && claim >= big_tiles * U + units - split) {
+            // half h of tail unit u: sub counts halves of the tile
+            const u64 h = claim - (big_tiles * U + units - split);
+            if (h >= 2 * split) break;
+            const u64 u = units - split + h / 2;
+            tile = big_tiles + u / U;
+            sub = (u % U) * 2 + (h & 1);
+            Keff = K / (2 * U);
+        }
         if (tile >= total_tiles) break;
         if (a.serve != nullptr && a.serve->n_active == 0) break;
         // claim number: whole tiles first, then the units of the tail
