@@ -1,9 +1,10 @@
 """Comb widths at the ends of the range mk_open accepts (2..28; the reference's is 2..20,
 src/controller/cli.rs:85-94) and past the production W = 24: W = 2 (126 rows of 3
 entries), W = 25 and 26 (23.6 and 43 GB tables, where address translation slows the
-gathers), and W = 28 (9 rows of 2^27 + 1 entries, a 154.6 GB table, the largest). Each
-must give the closed-form table entries, the oracle's public keys in both seed modes
-and the oracle's hit sets."""
+gathers), and W = 28 (9 rows of 2^27 + 1 entries, a 154.6 GB table, the largest), in the
+default ten-limb layout and, at W = 2, 25 and 28, in the Karatsuba layouts. Each must give
+the closed-form table entries, the oracle's public keys in both seed modes and the
+oracle's hit sets."""
 import os
 import random
 import sys
@@ -23,22 +24,34 @@ def say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("width", [2, 25, 26, 28])
-def test_width_extremes(oracle, width):
+def _entry_point_any(words, limbs):
+    """(x, y, third field) of a raw entry in any layout."""
+    from tests.test_gpu_layouts import _entry_values
+    if limbs == M.LIMBS_R25U:
+        return _entry_point(words)
+    ypx, ymx, third = _entry_values(words, limbs)
+    assert max(ypx, ymx, third) < T.P
+    return (ypx - ymx) % T.P, (ypx + ymx) % T.P, third
+
+
+@pytest.mark.parametrize("width,limbs", [(2, M.LIMBS_R25U), (25, M.LIMBS_R25U), (26, M.LIMBS_R25U),
+                                         (28, M.LIMBS_R25U), (2, M.LIMBS_K26U), (25, M.LIMBS_K26),
+                                         (28, M.LIMBS_K26U)])
+def test_width_extremes(oracle, width, limbs):
     rows, cols = -(-251 // width), (1 << (width - 1)) + 1
-    say(f"open gpu W={width} batch=32: {rows} rows x {cols} entries")
-    g = M.Gpu(0, width=width, batch=32)
+    say(f"open gpu W={width} batch=32 limbs={limbs}: {rows} rows x {cols} entries")
+    g = M.Gpu(0, width=width, batch=32, limbs=limbs)
     try:
         info = g.info()
         assert (info.rows, info.cols) == (rows, cols)
-        assert info.table_bytes == rows * cols * 128
-        rng = random.Random(600 + width)
+        assert info.table_bytes == rows * cols * 4 * M.ENTRY_WORDS[limbs]
+        rng = random.Random(600 + width + limbs)
         # entries: the compressed point k * 2^(w * row + 3) * B (+ 2^254 * B on the top row)
         # from the oracle's own double-and-add, the curve equation and the third field
         picks = [(0, 0), (0, 1), (0, cols - 1), (rows - 1, 0), (rows - 1, 1), (rows - 1, cols - 1)]
         picks += [(rng.randrange(rows), rng.randrange(cols)) for _ in range(10)]
         for row, k in picks:
-            x, y, third = _entry_point(g.table_words(row, k))
+            x, y, third = _entry_point_any(g.table_words(row, k), limbs)
             assert (y * y - x * x - 1 - T.D * x * x % T.P * y * y) % T.P == 0, (row, k)
             assert int.to_bytes(y | ((x & 1) << 255), 32, "little") == oracle.comb_entry(width, row, k), (row, k)
             assert third == (x * y % T.P if row == 0 else T.D * x % T.P * y % T.P), (row, k)
