@@ -10,6 +10,7 @@
 
 #include <vector>
 
+#include "mk_claims.h"
 #include "mk_field.h"
 #include "mk_hash.h"
 #include "mk_match.h"
@@ -99,6 +100,25 @@ void hc_fe_op(u32 op, const u32 a[8], const u32 b[8], u32 out[8]) {
         default: r = fe_neg(x); break;
     }
     fe_to_words(r, out);
+}
+
+// The claims of one launch in the order the counters hand them out: for each, the first key
+// (of the launch's 64 * K * total_tiles) and the keys per lane; a claim covers 64 * Keff
+// keys. Returns the number of claims, or -1 when they exceed cap.
+long hc_claims(u32 K, u32 serving, u64 total_tiles, u64 waves, u64* starts, u32* keffs, u64 cap) {
+    const mk_claim_plan p = mk_plan_claims(K, serving != 0, total_tiles, waves);
+    u64 n = 0, q = 0;
+    for (u64 t = 0;; t++) {
+        const u64 claim = mk_claim_number(p, t, t < p.big_tiles ? 0 : q++);
+        u64 tile, sub;
+        u32 keff;
+        if (!mk_claim_at(p, claim, tile, sub, keff)) break;
+        if (n == cap) return -1;
+        starts[n] = tile * 64 * K + sub * 64 * keff;
+        keffs[n] = keff;
+        n++;
+    }
+    return (long)n;
 }
 
 int hc_digits(const u32 s[8], u32 w, i32* out) {

@@ -42,6 +42,7 @@
 
 namespace MK_NS {
 
+#include "mk_claims.h"
 #include "mk_field.h"
 #include "mk_hash.h"
 #include "mk_match.h"
@@ -174,18 +175,8 @@ struct mk_wave_ctx {
     const u32* ltable;  // the comb table in LDS (the LDS-table kernel), else null
 };
 
-// Tail balancing: the last tiles (two per wave of the grid for a wave that claims on its
-// own, one per wave for workgroup claims) are cut into U units of K / U keys per lane,
-// claimed one by one from a second counter, so the waves finish within a fraction of a
-// tile of each other. Units keep at least 64 keys per lane (the inversion is shared by a
-// unit's keys); two tiles per wave in units measured 0.5% faster than one, four and
-// eight slower again, and 32-key units slower than 64. A serving step of one tile per
-// wave is therefore claimed entirely in units; there they are 128 keys, which balance
-// the step's mix of matchers at half the inversions of 64-key units (0.5% more served
-// keys/s; whole tiles serve 5% fewer).
-__device__ __forceinline__ u32 mk_units_per_tile(u32 K, bool serving) {
-    return serving ? ((K >= 128 && K % 128 == 0) ? K / 128 : 1) : ((K >= 128 && K % 64 == 0) ? K / 64 : 1);
-}
+// Tail balancing (the claim layout and its measurements): mk_claims.h. Workgroup claims
+// (the lockstep and pooled kernels below) cut one tile per wave of the tail into units.
 
 // One claim of one wave: tile `tile` (sub-unit `sub` of a tail tile) with Keff keys per
 // lane; nth is the claim's number in claim order (serving deals claims round-robin over
@@ -326,44 +317,22 @@ struct mk_lane_invert {
 
 // The search kernel: one wave per workgroup, each wave claims its own tiles.
 // The claim loop of a wave that claims on its own (`waves` = waves of the grid).
-#ifndef MK_TAIL_SPLIT
-#define MK_TAIL_SPLIT 1
-#endif
 template <bool LDST>
 __device__ __forceinline__ void mk_wave_claims(const mk_search_args& a, mk_wave_ctx& cx, u64 waves) {
-    const u32 K = a.K;
-    const u32 U = mk_units_per_tile(K, a.serve != nullptr);
-    const u64 total_tiles = a.total_tiles;
-    const u64 tail = U > 1 ? (total_tiles < 2 * waves ? total_tiles : 2 * waves) : 0;
-    const u64 big_tiles = total_tiles - tail;
-    // The last `split` units of the tail (one per wave of the grid, the final round of
-    // claims) are cut once more into halves of K / (2U) keys per lane. Measured on one box
-    // each: 0.3-0.6% faster than without (three and two A/B pairs); halving the last two
-    // rounds, or quartering the last round's halves, no faster again.
-    const u64 units = tail * U;
-    const u64 split = (MK_TAIL_SPLIT && a.serve == nullptr && U > 1 && units >= 2 * waves) ? waves : 0;
+    const mk_claim_plan plan = mk_plan_claims(a.K, a.serve != nullptr, a.total_tiles, waves);
     for (;;) {
         u64 claim = 0;
         if (cx.lane == 0) {
             const u64 t = atomicAdd((unsigned long long*)&a.ctr->queue, 1ull);
-            claim = t < big_tiles ? t * U : big_tiles * U + atomicAdd((unsigned long long*)&a.ctr->queue2, 1ull);
+            claim = mk_claim_number(plan, t, t < plan.big_tiles ? 0 : atomicAdd((unsigned long long*)&a.ctr->queue2, 1ull));
         }
         claim = mk_uniform64(claim);
-        u64 tile = claim / U, sub = claim % U;
-        u32 Keff = tile >= big_tiles ? K / U : K;
-        if (split != 0 && claim >= big_tiles * U + units - split) {
-            // half h of tail unit u: sub counts halves of the tile
-            const u64 h = claim - (big_tiles * U + units - split);
-            if (h >= 2 * split) break;
-            const u64 u = units - split + h / 2;
-            tile = big_tiles + u / U;
-            sub = (u % U) * 2 + (h & 1);
-            Keff = K / (2 * U);
-        }
-        if (tile >= total_tiles) break;
+        u64 tile, sub;
+        u32 Keff;
+        if (!mk_claim_at(plan, claim, tile, sub, Keff)) break;
         if (a.serve != nullptr && a.serve->n_active == 0) break;
         // claim number: whole tiles first, then the units of the tail
-        const u64 nth = tile < big_tiles ? tile : big_tiles + (tile - big_tiles) * U + sub;
+        const u64 nth = tile < plan.big_tiles ? tile : plan.big_tiles + (tile - plan.big_tiles) * plan.U + sub;
         mk_search_claim<LDST>(a, cx, tile, sub, Keff, nth, [] {}, mk_lane_invert());
     }
 }

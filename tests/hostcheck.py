@@ -57,6 +57,8 @@ def _bind(L):
     L.hc_fe_carried_to_words.argtypes = [P, P]
     L.hc_fe_op_limbs.argtypes = [U32, P, P, P]
     L.hc_layout.restype = ctypes.c_int
+    L.hc_claims.argtypes = [U32, U32, U64, U64, P, P, U64]
+    L.hc_claims.restype = ctypes.c_long
     return L
 
 

@@ -309,3 +309,40 @@ def test_hit_order(hostcheck, case):
     assert sorted(order2.tolist()) == list(range(len(s2)))
     want2 = np.lexsort((x2, s2))
     assert s2[order2].tolist() == s2[want2].tolist() and x2[order2].tolist() == x2[want2].tolist()
+
+
+@pytest.mark.parametrize("K", [1, 8, 64, 128, 192, 256])
+def test_claims_cover_every_key_once(hostcheck, K):
+    """The search kernel's claim layout (mk_claims.h): whole tiles, tail units, the halved
+    final round. For launches of every size around the tail's boundaries, the claims the
+    counters hand out cover the launch's 64 * K * tiles keys exactly once, halves
+    (K / 2U keys per lane) appear only in the last round of the tail, one per wave, and
+    serving steps are cut into 128-key units and never halved."""
+    import numpy as np
+    waves = 64
+    for serving in (0, 1):
+        U = (K // 128 if K >= 128 and K % 128 == 0 else 1) if serving else (K // 64 if K >= 128 and K % 64 == 0 else 1)
+        for tiles in [0, 1, 2, 5, waves - 1, waves, waves + 1, 2 * waves - 1, 2 * waves, 2 * waves + 1, 3 * waves + 7,
+                      10 * waves]:
+            cap = tiles * 2 * max(U, 1) + 8
+            starts = np.zeros(cap, np.uint64)
+            keffs = np.zeros(cap, np.uint32)
+            n = hostcheck.lib().hc_claims(K, serving, tiles, waves, starts.ctypes.data, keffs.ctypes.data, cap)
+            assert n >= 0, (K, serving, tiles)
+            s, k = starts[:n].astype(np.int64), keffs[:n].astype(np.int64)
+            order = np.argsort(s, kind="stable")
+            ends = s[order] + 64 * k[order]
+            if n:
+                assert s[order][0] == 0 and ends[-1] == 64 * K * tiles, (K, serving, tiles)
+                assert (s[order][1:] == ends[:-1]).all(), (K, serving, tiles)  # no gap, no overlap
+            else:
+                assert tiles == 0
+            halves = int((k < K // max(U, 1)).sum())
+            if serving or U == 1:
+                assert halves == 0
+                assert set(k.tolist()) <= {K, K // max(U, 1)}
+            elif tiles * U >= 2 * waves:
+                # the final round: the last `waves` units of the tail, each as two halves
+                assert halves == 2 * waves and (k[-2 * waves:] == K // (2 * U)).all(), (K, tiles)
+            else:
+                assert halves == 0
