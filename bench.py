@@ -283,7 +283,7 @@ def serving_leg(gpu, seconds, step_keys=1 << 24):
     from meikey_amd import workloads as W
     reqs = W.serving(50.0, seconds)
     srv = M.Server(gpu, max_slots=4096, use_graph=True, dfa_budget=256 << 20, step_keys=step_keys,
-                   active_per_step=64)
+                   active_per_step=16)
     ids, results, i = {}, {}, 0
     t0 = srv.now()
     steps = 0
@@ -318,7 +318,7 @@ def serving_leg(gpu, seconds, step_keys=1 << 24):
            "p99_censored": bool(censored) and p99 is not None and p99 >= min(censored),
            "p50_found_only_s": percentile(ttfh, 0.5), "p99_found_only_s": percentile(ttfh, 0.99),
            "rate_per_s": 50.0,
-           "policy": f"least attained service, 64 requests per step, {step_keys} candidates per step, two steps "
+           "policy": f"least attained service, 16 requests per step, {step_keys} candidates per step, two steps "
                      "in flight on two streams serving disjoint requests",
            "censoring": "requests unfinished at the end count with their elapsed time (lower bound)"}
     print(f"# serving: {len(found)}/{len(ids)} found, p50 {out['p50_time_to_first_hit_s']:.3f} s, "

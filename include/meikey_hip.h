@@ -218,7 +218,8 @@ typedef struct {
     uint64_t step_keys;       /* candidates per step (rounded up to whole tiles; at least one claim unit,
                                  128 keys per lane or a whole smaller tile, per wave) */
     uint32_t active_per_step; /* resident requests served per step, the ones with the fewest
-                                 candidates searched so far (least attained service); 0 = all */
+                                 candidates searched so far (least attained service); 0 = all;
+                                 default 16 */
     uint32_t overlap;         /* 1: two steps in flight (disjoint requests); 0: one step at a time */
 } mk_server_options;
 

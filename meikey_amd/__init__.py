@@ -565,7 +565,7 @@ class Server:
     """Continuous batching of search requests on one Gpu (mk_server_* of the C ABI)."""
 
     def __init__(self, gpu: Gpu, max_slots: int = 1024, use_graph: bool = True, dfa_budget: int = 64 << 20,
-                 step_keys: int = 1 << 26, active_per_step: int = 64, overlap: bool = True):
+                 step_keys: int = 1 << 26, active_per_step: int = 16, overlap: bool = True):
         self.gpu = gpu
         opt = ServerOptions(max_slots, int(use_graph), dfa_budget, step_keys, active_per_step, int(overlap))
         h = ctypes.c_void_p()
