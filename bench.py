@@ -171,6 +171,8 @@ def main():
     served = []
     if args.serve_seconds > 0:
         out["serving"], served = serving_leg(gpu, args.serve_seconds)
+        # the same arrivals served in steps half as long (64-key claim units): lower latency
+        out["serving_low_latency"], _ = serving_leg(gpu, min(args.serve_seconds, 20.0), 1 << 23, 64)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(gpu, key, needle, served, low, args.cpu_seconds)
     gpu.close()
@@ -274,7 +276,7 @@ def percentile(xs, q):
     return xs[k]
 
 
-def serving_leg(gpu, seconds, step_keys=1 << 24):
+def serving_leg(gpu, seconds, step_keys=1 << 24, unit_keys=128):
     """Poisson arrivals at 50/s for `seconds`, pattern length 3-6, kinds mixed 25% each,
     every request stopping at its first hit. Time to first hit = done - submit. Requests
     unfinished when the arrivals stop are cancelled and counted with their elapsed time,
@@ -283,7 +285,7 @@ def serving_leg(gpu, seconds, step_keys=1 << 24):
     from meikey_amd import workloads as W
     reqs = W.serving(50.0, seconds)
     srv = M.Server(gpu, max_slots=4096, use_graph=True, dfa_budget=256 << 20, step_keys=step_keys,
-                   active_per_step=16)
+                   active_per_step=16, unit_keys=unit_keys)
     ids, results, i = {}, {}, 0
     t0 = srv.now()
     steps = 0
@@ -318,10 +320,11 @@ def serving_leg(gpu, seconds, step_keys=1 << 24):
            "p99_censored": bool(censored) and p99 is not None and p99 >= min(censored),
            "p50_found_only_s": percentile(ttfh, 0.5), "p99_found_only_s": percentile(ttfh, 0.99),
            "rate_per_s": 50.0,
-           "policy": f"least attained service, 16 requests per step, {step_keys} candidates per step, two steps "
-                     "in flight on two streams serving disjoint requests",
+           "policy": f"least attained service, 16 requests per step, {step_keys} candidates per step claimed in "
+                     f"units of {unit_keys} keys per lane, two steps in flight on two streams serving disjoint "
+                     "requests",
            "censoring": "requests unfinished at the end count with their elapsed time (lower bound)"}
-    print(f"# serving: {len(found)}/{len(ids)} found, p50 {out['p50_time_to_first_hit_s']:.3f} s, "
+    print(f"# serving ({unit_keys}-key units): {len(found)}/{len(ids)} found, p50 {out['p50_time_to_first_hit_s']:.3f} s, "
           f"p99 {p99:.3f} s, {tries / end / 1e9:.3f} G keys/s", file=sys.stderr, flush=True)
     served = [(reqs[k], r) for k, r in results.items() if r.status == 1]
     return out, served

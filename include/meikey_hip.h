@@ -216,11 +216,16 @@ typedef struct {
     uint32_t use_graph;       /* 1: replay the captured step graph; 0: enqueue each step */
     uint64_t dfa_budget;      /* HBM bytes for the admitted requests' DFA tables */
     uint64_t step_keys;       /* candidates per step (rounded up to whole tiles; at least one claim unit,
-                                 128 keys per lane or a whole smaller tile, per wave) */
+                                 unit_keys keys per lane or a whole smaller tile, per wave) */
     uint32_t active_per_step; /* resident requests served per step, the ones with the fewest
                                  candidates searched so far (least attained service); 0 = all;
                                  default 16 */
     uint32_t overlap;         /* 1: two steps in flight (disjoint requests); 0: one step at a time */
+    uint32_t unit_keys;       /* keys per lane of a step's claim unit: 64, 128 (the default; 0 means
+                                 128), 192 or 256; one that does not divide the batch leaves tiles whole.
+                                 64 with half the step_keys halves a step and the median time to a
+                                 hit, at about 1% of the served keys/s */
+    uint32_t pad;
 } mk_server_options;
 
 typedef struct {

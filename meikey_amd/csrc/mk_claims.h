@@ -13,13 +13,28 @@
 // final halves, gained nothing more. A serving step of one tile per wave is claimed
 // entirely in units; there they are 128 keys, which balance the step's mix of matchers
 // at half the inversions of 64-key units (0.5% more served keys/s; whole tiles serve 5%
-// fewer), and they are not halved (two overlapping steps fill each other's tails).
+// fewer), and they are not halved (two overlapping steps fill each other's tails). A server
+// may ask for 64-key units instead, so that a step of one unit per wave is half as long.
 #pragma once
 
 #include "mk_common.h"
 
-MK_DEV u32 mk_units_per_tile(u32 K, bool serving) {
-    return serving ? ((K >= 128 && K % 128 == 0) ? K / 128 : 1) : ((K >= 128 && K % 64 == 0) ? K / 64 : 1);
+// Host and device code both plan a launch's claims (the server sizes its steps with them).
+#if defined(__HIPCC__)
+#define MK_CLAIMS_FN __host__ __device__ __forceinline__
+#else
+#define MK_CLAIMS_FN static inline
+#endif
+
+// Keys per lane of a serving step's claim unit unless the server asks for another size.
+#define MK_SERVE_UNIT_KEYS 128
+
+// Units a tile of K keys per lane is cut into: serve_unit keys per lane in a serving step
+// (serve_unit != 0; a unit that does not divide K leaves the tile whole), 64 keys per lane
+// in the tail of a search launch.
+MK_CLAIMS_FN u32 mk_units_per_tile(u32 K, u32 serve_unit) {
+    return serve_unit != 0 ? ((K >= serve_unit && K % serve_unit == 0) ? K / serve_unit : 1)
+                           : ((K >= 128 && K % 64 == 0) ? K / 64 : 1);
 }
 
 // The claim layout of one launch: claims [0, big_tiles * U) are whole tiles (claim / U,
@@ -30,10 +45,11 @@ struct mk_claim_plan {
     u64 total_tiles, big_tiles, units, split;
 };
 
-MK_DEV mk_claim_plan mk_plan_claims(u32 K, bool serving, u64 total_tiles, u64 waves) {
+MK_CLAIMS_FN mk_claim_plan mk_plan_claims(u32 K, u32 serve_unit, u64 total_tiles, u64 waves) {
+    const bool serving = serve_unit != 0;
     mk_claim_plan p;
     p.K = K;
-    p.U = mk_units_per_tile(K, serving);
+    p.U = mk_units_per_tile(K, serve_unit);
     p.total_tiles = total_tiles;
     const u64 tail = p.U > 1 ? (total_tiles < 2 * waves ? total_tiles : 2 * waves) : 0;
     p.big_tiles = total_tiles - tail;
@@ -44,13 +60,13 @@ MK_DEV mk_claim_plan mk_plan_claims(u32 K, bool serving, u64 total_tiles, u64 wa
 
 // The claim number handed out for the t-th draw of the first counter (t < big_tiles: a
 // whole tile) or, past it, the q-th draw of the second counter.
-MK_DEV u64 mk_claim_number(const mk_claim_plan& p, u64 t, u64 q) {
+MK_CLAIMS_FN u64 mk_claim_number(const mk_claim_plan& p, u64 t, u64 q) {
     return t < p.big_tiles ? t * p.U : p.big_tiles * p.U + q;
 }
 
 // Tile, sub-range and keys per lane of a claim: the claim covers keys
 // [tile * 64 * K + sub * 64 * Keff, + 64 * Keff) of the launch. False past the last claim.
-MK_DEV bool mk_claim_at(const mk_claim_plan& p, u64 claim, u64& tile, u64& sub, u32& Keff) {
+MK_CLAIMS_FN bool mk_claim_at(const mk_claim_plan& p, u64 claim, u64& tile, u64& sub, u32& Keff) {
     const u32 U = p.U;
     tile = claim / U;
     sub = claim % U;

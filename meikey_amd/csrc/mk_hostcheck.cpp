@@ -104,9 +104,10 @@ void hc_fe_op(u32 op, const u32 a[8], const u32 b[8], u32 out[8]) {
 
 // The claims of one launch in the order the counters hand them out: for each, the first key
 // (of the launch's 64 * K * total_tiles) and the keys per lane; a claim covers 64 * Keff
-// keys. Returns the number of claims, or -1 when they exceed cap.
-long hc_claims(u32 K, u32 serving, u64 total_tiles, u64 waves, u64* starts, u32* keffs, u64 cap) {
-    const mk_claim_plan p = mk_plan_claims(K, serving != 0, total_tiles, waves);
+// keys. serve_unit: keys per lane of a serving step's unit, 0 for a search launch. Returns
+// the number of claims, or -1 when they exceed cap.
+long hc_claims(u32 K, u32 serve_unit, u64 total_tiles, u64 waves, u64* starts, u32* keffs, u64 cap) {
+    const mk_claim_plan p = mk_plan_claims(K, serve_unit, total_tiles, waves);
     u64 n = 0, q = 0;
     for (u64 t = 0;; t++) {
         const u64 claim = mk_claim_number(p, t, t < p.big_tiles ? 0 : q++);

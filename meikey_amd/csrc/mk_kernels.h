@@ -45,11 +45,12 @@ struct mk_hit {
 
 // Serving: the active request slots of a step. Tile t of the step goes to slot
 // slots[t % n_active]; the slot's next index (next_index[slot], device-resident) is
-// advanced atomically by each tile it receives.
+// advanced atomically by each tile it receives. The step is claimed in units of
+// unit_keys keys per lane.
 #define MK_SERVE_MAX_SLOTS 4096
 struct mk_serve_state {
     u32 n_active;
-    u32 pad;
+    u32 unit_keys;
     u32 slots[MK_SERVE_MAX_SLOTS];
 };
 

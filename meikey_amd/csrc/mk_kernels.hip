@@ -319,7 +319,8 @@ struct mk_lane_invert {
 // The claim loop of a wave that claims on its own (`waves` = waves of the grid).
 template <bool LDST>
 __device__ __forceinline__ void mk_wave_claims(const mk_search_args& a, mk_wave_ctx& cx, u64 waves) {
-    const mk_claim_plan plan = mk_plan_claims(a.K, a.serve != nullptr, a.total_tiles, waves);
+    const u32 serve_unit = a.serve != nullptr ? __builtin_amdgcn_readfirstlane(a.serve->unit_keys) : 0u;
+    const mk_claim_plan plan = mk_plan_claims(a.K, serve_unit, a.total_tiles, waves);
     for (;;) {
         u64 claim = 0;
         if (cx.lane == 0) {
@@ -459,7 +460,7 @@ __device__ __forceinline__ void mk_search_lock(const mk_search_args& a) {
     cx.smid = &smid[wave];
     cx.ltable = nullptr;
     const u32 K = a.K;
-    const u32 U = mk_units_per_tile(K, a.serve != nullptr);
+    const u32 U = mk_units_per_tile(K, a.serve != nullptr ? __builtin_amdgcn_readfirstlane(a.serve->unit_keys) : 0u);
     const u64 total_tiles = a.total_tiles;
     const u64 waves = (u64)gridDim.x * nw;
     // whole tiles come in groups of nw, so a group never straddles the tail

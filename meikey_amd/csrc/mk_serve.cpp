@@ -36,6 +36,7 @@
 #include <string>
 #include <vector>
 
+#include "mk_claims.h"
 #include "mk_internal.h"
 
 #define MK_MATCHER_FAIL(st, ...) (mk_fail((mk_status)(st), __VA_ARGS__), (st))
@@ -338,6 +339,7 @@ void mk_server_default_options(mk_server_options* out) {
     out->step_keys = 1ull << 26;
     out->active_per_step = 16;
     out->overlap = 1;
+    out->unit_keys = MK_SERVE_UNIT_KEYS;
 }
 
 mk_status mk_server_open(mk_gpu* g, const mk_server_options* opt_in, mk_server** out) {
@@ -348,14 +350,17 @@ mk_status mk_server_open(mk_gpu* g, const mk_server_options* opt_in, mk_server**
     if (opt_in) opt = *opt_in;
     if (opt.max_slots < 1 || opt.max_slots > MK_SERVE_MAX_SLOTS)
         return mk_fail(MK_ERR_OUT_OF_RANGE, "max_slots must be 1-%d (got %u)", MK_SERVE_MAX_SLOTS, opt.max_slots);
+    if (opt.unit_keys == 0) opt.unit_keys = MK_SERVE_UNIT_KEYS;
+    if (opt.unit_keys % 64 != 0 || opt.unit_keys > 256)
+        return mk_fail(MK_ERR_OUT_OF_RANGE, "unit_keys must be 64, 128, 192 or 256 (got %u)", opt.unit_keys);
     MK_HIPS(hipSetDevice(g->device));
     mk_server* s = new mk_server();
     s->g = g;
     s->opt = opt;
     const u64 tile_keys = 64ull * g->opt.batch;
-    // The search kernel claims a serving step in units of 128 keys per lane (tiles of
-    // 256 keys per lane are two units), so a step needs one unit per wave at least.
-    const u64 units_per_tile = (g->opt.batch >= 128 && g->opt.batch % 128 == 0) ? g->opt.batch / 128 : 1;
+    // The search kernel claims a serving step in units of unit_keys keys per lane (a tile
+    // of 256 keys per lane is two 128-key units), so a step needs one unit per wave at least.
+    const u64 units_per_tile = mk_units_per_tile(g->opt.batch, opt.unit_keys);
     const u64 min_tiles = (g->grid + units_per_tile - 1) / units_per_tile;
     s->step_tiles = std::max<u64>(min_tiles, (opt.step_keys + tile_keys - 1) / tile_keys);
     s->slots.resize(opt.max_slots);
@@ -397,6 +402,7 @@ mk_status mk_server_open(mk_gpu* g, const mk_server_options* opt_in, mk_server**
     }
     for (step_buf& B : s->buf) {
         memset(B.h_serve, 0, sizeof(mk_serve_state));
+        B.h_serve->unit_keys = opt.unit_keys;
         memset(B.h_ctr, 0, sizeof(mk_counters));
         memset(B.h_next, 0, sizeof(u64) * opt.max_slots);
     }

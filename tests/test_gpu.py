@@ -574,13 +574,15 @@ def _serve_specs(n, seed):
     return out
 
 
-@pytest.mark.parametrize("use_graph,overlap", [(True, True), (False, True), (True, False)])
-def test_server_results_alone_and_in_the_mix(gpu, oracle, use_graph, overlap):
+@pytest.mark.parametrize("use_graph,overlap,unit", [(True, True, 128), (False, True, 128), (True, False, 128),
+                                                   (True, True, 64), (False, False, 64)])
+def test_server_results_alone_and_in_the_mix(gpu, oracle, use_graph, overlap, unit):
     """Continuous batching: requests submitted while others run; each result equals the
     result of the same request served alone, and the oracle confirms it. With overlap, two
-    steps serving disjoint requests are in flight on two streams."""
+    steps serving disjoint requests are in flight on two streams. Steps claimed in 64-key
+    units (half the default) give the same results."""
     specs = _serve_specs(24, 7)
-    with M.Server(gpu, max_slots=8, use_graph=use_graph, step_keys=1 << 22, overlap=overlap) as srv:
+    with M.Server(gpu, max_slots=8, use_graph=use_graph, step_keys=1 << 22, overlap=overlap, unit_keys=unit) as srv:
         ids = {}
         results = {}
         i = 0
@@ -600,7 +602,8 @@ def test_server_results_alone_and_in_the_mix(gpu, oracle, use_graph, overlap):
         _served_checks(oracle, s, results[k])
     # alone
     for k in (0, 5, 11, 23):
-        with M.Server(gpu, max_slots=4, use_graph=use_graph, step_keys=1 << 22, overlap=overlap) as srv:
+        with M.Server(gpu, max_slots=4, use_graph=use_graph, step_keys=1 << 21, overlap=overlap,
+                      unit_keys=unit) as srv:
             rid = srv.submit(specs[k])
             got = []
             while not got:
@@ -624,6 +627,18 @@ def test_server_graph_replays_match_eager(gpu):
                     res[ids[r.request]] = (r.status, r.index, r.pk)
             out[use_graph] = res
     assert out[True] == out[False]
+
+
+def test_server_unit_keys_checked(gpu):
+    """unit_keys is 64, 128, 192 or 256 (0 means the default 128); other values are refused
+    before anything is allocated."""
+    for bad in (32, 96, 320):
+        with pytest.raises(M.MeikeyError) as e:
+            M.Server(gpu, max_slots=2, unit_keys=bad)
+        assert e.value.status == 3  # MK_ERR_OUT_OF_RANGE
+    for ok in (0, 192, 256):
+        with M.Server(gpu, max_slots=2, step_keys=1 << 20, unit_keys=ok) as srv:
+            assert srv.stats()[1] >= 1 << 20
 
 
 def test_server_budget_and_cancel(gpu):

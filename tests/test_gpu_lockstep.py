@@ -98,7 +98,8 @@ def test_lockstep_stop_on_hit(pair, oracle):
 
 def test_lockstep_serving(pair, oracle):
     """Served requests (one 128-key unit per wave per step, round-robin over requests,
-    stopped requests skipped) give each request's first exact hit, as without lockstep."""
+    stopped requests skipped) give each request's first exact hit, as without lockstep,
+    also in 64-key units."""
     lock, plain = pair
     rng = random.Random(163)
     specs = []
@@ -107,9 +108,9 @@ def test_lockstep_serving(pair, oracle):
         p = M.Pattern(text, i % 2 == 0, M.MODE_SUFFIX if i % 3 == 0 else M.MODE_CONTAINS)
         specs.append(p.spec(rng.randbytes(32), rng.getrandbits(40), 0))
     results = {}
-    for name, g in (("lock", lock), ("plain", plain)):
+    for name, g, unit in (("lock", lock, 128), ("lock64", lock, 64), ("plain", plain, 128)):
         say(f"serving on {name}")
-        with M.Server(g, max_slots=8, step_keys=1 << 22) as srv:
+        with M.Server(g, max_slots=8, step_keys=1 << 22, unit_keys=unit) as srv:
             ids = {srv.submit(s): k for k, s in enumerate(specs)}
             res = {}
             steps = 0
@@ -120,7 +121,7 @@ def test_lockstep_serving(pair, oracle):
                     res[ids[r.request]] = (r.status, r.index, r.pk)
                 assert steps < 4000
         results[name] = res
-    assert results["lock"] == results["plain"]
+    assert results["lock"] == results["plain"] == results["lock64"]
     for k, s in enumerate(specs):
         status, index, pk = results["lock"][k]
         assert status == 1

@@ -317,11 +317,15 @@ def test_claims_cover_every_key_once(hostcheck, K):
     final round. For launches of every size around the tail's boundaries, the claims the
     counters hand out cover the launch's 64 * K * tiles keys exactly once, halves
     (K / 2U keys per lane) appear only in the last round of the tail, one per wave, and
-    serving steps are cut into 128-key units and never halved."""
+    serving steps are cut into units of the server's size (128 or 64 keys per lane) and
+    never halved."""
     import numpy as np
     waves = 64
-    for serving in (0, 1):
-        U = (K // 128 if K >= 128 and K % 128 == 0 else 1) if serving else (K // 64 if K >= 128 and K % 64 == 0 else 1)
+    for serving in (0, 128, 64):
+        if serving:
+            U = K // serving if K >= serving and K % serving == 0 else 1
+        else:
+            U = K // 64 if K >= 128 and K % 64 == 0 else 1
         for tiles in [0, 1, 2, 5, waves - 1, waves, waves + 1, 2 * waves - 1, 2 * waves, 2 * waves + 1, 3 * waves + 7,
                       10 * waves]:
             cap = tiles * 2 * max(U, 1) + 8
