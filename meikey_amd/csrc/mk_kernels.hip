@@ -268,16 +268,18 @@ __device__ __forceinline__ bool mk_search_claim(const mk_search_args& a, mk_wave
             mt.dfa = ldfa;
         }
     };
-    // volatile: loaded where each key's hash needs it, not kept in registers
+    // Loaded where each key's hash needs it, not kept in registers through the comb: the
+    // empty asm with a memory clobber keeps the compiler from hoisting the LDS reads out
+    // of the key loop. (A volatile pointer did the same through serialized flat loads.)
     const mk_sha_mid* smid = cx.smid;
     auto mid = [&]() {
-        const volatile mk_sha_mid* v = smid;
+        asm volatile("" ::: "memory");
         mk_sha_mid m;
         MK_UNROLL
-        for (int i = 0; i < 8; i++) m.v[i] = v->v[i];
-        m.w2 = v->w2;
-        m.w16 = v->w16;
-        m.w17 = v->w17;
+        for (int i = 0; i < 8; i++) m.v[i] = smid->v[i];
+        m.w2 = smid->w2;
+        m.w16 = smid->w16;
+        m.w17 = smid->w17;
         return m;
     };
     mk_counters* ctr = a.ctr;
