@@ -5,6 +5,7 @@ hit sets. Sizes are chosen so the oracle finishes in seconds; the 2^24 config-1 
 is compared in full.
 """
 import json
+import math
 import os
 import random
 import sys
@@ -154,21 +155,29 @@ def test_search_vs_oracle(gpu, oracle, source, needle, cs, mode):
         assert [h.seed for h in hits if h.index == i][0] == oracle.seed(key, source, i)
 
 
-def test_tail_units_match_whole_tiles(gpu, oracle):
-    """At 256 keys per lane the last tile per wave is cut into 64-key units claimed from a
-    second counter; hit sets over the same ranges equal those of whole 32-key tiles (and
-    the oracle on the smaller searches). The first search spans more tiles than the grid
-    has waves, so it runs through both the whole-tile and the unit claims."""
-    say("open gpu W=16 batch=256 blocks=1")
-    big = M.Gpu(0, width=16, batch=256, blocks=1)
+@pytest.mark.parametrize("batch", [256, 192, 128, 200])
+def test_tail_units_match_whole_tiles(gpu, oracle, batch):
+    """The last two tiles per wave of a launch are cut into 64-key units claimed from a
+    second counter (4, 3 and 2 units at 256, 192 and 128 keys per lane; a batch that is no
+    multiple of 64 keeps whole tiles), and the final round of units is claimed as halves.
+    The launch spans more than two tiles per wave of the grid, so it runs through the
+    whole-tile claims, the units and the halves. Hit sets over the same ranges equal those
+    of whole 32-key tiles, and the oracle's on the smaller searches."""
+    say(f"open gpu W=16 batch={batch} blocks=1")
+    big = M.Gpu(0, width=16, batch=batch, blocks=1)
     try:
-        tile = 64 * 256
-        assert big.info().grid_waves * tile <= (1 << 24)
-        rng = random.Random(77)
+        tile = 64 * batch
+        step = tile * 2048 // math.gcd(tile, 2048)  # whole tiles of both kernels
+
+        def span(n):
+            return -(-n // step) * step
+
+        waves = big.info().grid_waves
+        rng = random.Random(77 + batch)
         specs = []
-        for count, needle, cs, mode in [((1 << 24) + 3 * tile, b"Kq7", True, M.MODE_CONTAINS),
-                                        (1 << 20, b"zz", False, M.MODE_SUFFIX),
-                                        (5 * tile, b"mk", False, M.MODE_CONTAINS)]:
+        for count, needle, cs, mode in [(span((2 * waves + 3) * tile), b"Kq7", True, M.MODE_CONTAINS),
+                                        (span(1 << 20), b"zz", False, M.MODE_SUFFIX),
+                                        (span(5 * tile), b"mk", False, M.MODE_CONTAINS)]:
             specs.append(_spec(rng.randbytes(32), rng.choice([0, 1]), needle, cs, mode, rng.getrandbits(40), count))
         hits, tries = big.batch_run(specs)
         assert tries == [s["count"] for s in specs]

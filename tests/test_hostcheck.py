@@ -311,7 +311,7 @@ def test_hit_order(hostcheck, case):
     assert s2[order2].tolist() == s2[want2].tolist() and x2[order2].tolist() == x2[want2].tolist()
 
 
-@pytest.mark.parametrize("K", [1, 8, 64, 128, 192, 256])
+@pytest.mark.parametrize("K", [1, 8, 64, 128, 192, 200, 256])
 def test_claims_cover_every_key_once(hostcheck, K):
     """The search kernel's claim layout (mk_claims.h): whole tiles, tail units, the halved
     final round. For launches of every size around the tail's boundaries, the claims the
