@@ -517,8 +517,9 @@ class Gpu:
         for i, s in enumerate(specs):
             _fill_spec(arr[i], s, keep)
         if out is not None:
-            if out.dtype != hit_dtype() or not out.flags["C_CONTIGUOUS"] or not out.flags["WRITEABLE"]:
-                raise ValueError("out must be a writeable C-contiguous array of hit_dtype()")
+            if (out.ndim != 1 or out.dtype != hit_dtype() or not out.flags["C_CONTIGUOUS"]
+                    or not out.flags["WRITEABLE"]):
+                raise ValueError("out must be a writeable, C-contiguous, one-dimensional array of hit_dtype()")
             cap = len(out)
             recs = ctypes.cast(out.ctypes.data, ctypes.POINTER(HitRec)) if cap else None
         else:

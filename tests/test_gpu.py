@@ -285,8 +285,10 @@ def test_batch_run_into_caller_buffer(gpu):
     with pytest.raises(M.MeikeyError) as e:
         gpu.batch_run(specs, out=np.zeros(40, dtype=M.hit_dtype()))
     assert e.value.status == 7  # MK_ERR_OVERFLOW
-    with pytest.raises(ValueError):
-        gpu.batch_run(specs, out=np.zeros(8, dtype=np.uint64))
+    for bad in (np.zeros(8, dtype=np.uint64), np.zeros((4, 100), dtype=M.hit_dtype()),
+                np.zeros(200, dtype=M.hit_dtype())[::2]):
+        with pytest.raises(ValueError):
+            gpu.batch_run(specs, out=bad)
 
 
 def test_config1_full_hit_set(gpu, oracle):
