@@ -119,3 +119,38 @@ def test_w24_table_entries_closed_form(gpu_default, oracle):
         assert (y * y - x * x - 1 - T.D * x * x % T.P * y * y) % T.P == 0, (row, k)
         assert int.to_bytes(y | ((x & 1) << 255), 32, "little") == oracle.comb_entry(24, row, k), (row, k)
         assert third == (x * y % T.P if row == 0 else T.D * x % T.P * y % T.P), (row, k)
+
+
+def test_full_range_hit_sets_of_two_field_layouts(gpu_default, oracle):
+    """The headline range at its full size: all 2^32 ChaCha20 candidates of the headline
+    key, searched for a 3-char case-sensitive suffix and a 4-char case-insensitive
+    substring, by the default ten-limb kernel ("10u") and by the eight-word layout ("8"),
+    whose field arithmetic shares no code with it. The two complete hit sets (index, seed,
+    public key) are equal; every 97th hit is re-derived by the oracle and matched as text;
+    the suffix count lies within 6 sigma of 2^32 / 64^3 = 16384 (sigma 128)."""
+    key, _ = W.single()
+    specs = [{"key": key, "seed_source": M.SEED_CHACHA20, "needle": b"Zq9", "fold": False, "mode": M.MODE_SUFFIX,
+              "first": 0, "count": 1 << 32},
+             {"key": key, "seed_source": M.SEED_CHACHA20, "needle": b"meek", "fold": True, "mode": M.MODE_CONTAINS,
+              "first": 0, "count": 1 << 32}]
+    say("full 2^32 range on the default layout")
+    hits, tries = gpu_default.batch_run(specs)
+    assert tries == [1 << 32, 1 << 32]
+    say("open gpu W=24 batch=256 limbs=8")
+    other = M.Gpu(0, width=24, batch=256, limbs=M.LIMBS_R32)
+    try:
+        say("full 2^32 range on the eight-word layout")
+        hits8, tries8 = other.batch_run(specs)
+    finally:
+        other.close()
+    assert tries8 == tries
+    suffix = [h for h in hits if h.search == 0]
+    substring = [h for h in hits if h.search == 1]
+    say(f"hits: suffix {len(suffix)}, substring {len(substring)}")
+    assert abs(len(suffix) - 16384) <= 6 * 128
+    assert len(substring) > 100000
+    assert [(h.search, h.index, h.seed, h.pk) for h in hits] == [(h.search, h.index, h.seed, h.pk) for h in hits8]
+    for h in suffix[::97] + substring[::97]:
+        assert h.pk == oracle.derive(key, 0, h.index, 1)
+        hay = T.hay(h.pk)
+        assert hay.endswith("Zq9") if h.search == 0 else "meek" in hay.lower()
