@@ -121,25 +121,29 @@ def test_w24_table_entries_closed_form(gpu_default, oracle):
         assert third == (x * y % T.P if row == 0 else T.D * x % T.P * y % T.P), (row, k)
 
 
-def test_full_range_hit_sets_of_two_field_layouts(gpu_default, oracle):
-    """The headline range at its full size: all 2^32 ChaCha20 candidates of the headline
-    key, searched for a 3-char case-sensitive suffix and a 4-char case-insensitive
-    substring, by the default ten-limb kernel ("10u") and by the eight-word layout ("8"),
-    whose field arithmetic shares no code with it. The two complete hit sets (index, seed,
-    public key) are equal; every 97th hit is re-derived by the oracle and matched as text;
-    the suffix count lies within 6 sigma of 2^32 / 64^3 = 16384 (sigma 128)."""
+@pytest.mark.parametrize("source,limbs", [(M.SEED_CHACHA20, M.LIMBS_R32), (M.SEED_COUNTER, M.LIMBS_K26U)],
+                         ids=["secure-8", "low-security-10ku"])
+def test_full_range_hit_sets_of_two_field_layouts(gpu_default, oracle, source, limbs):
+    """The headline range at its full size: all 2^32 candidates of the headline key
+    (ChaCha20 seeds, as the headline search; counter seeds, as the low-security one),
+    searched for a 3-char case-sensitive suffix and a 4-char case-insensitive substring,
+    by the default ten-limb kernel ("10u") and by a layout whose field arithmetic shares
+    no code with it (the eight-word "8"; the Karatsuba "10ku"). The two complete hit sets
+    (index, seed, public key) are equal; every 97th hit is re-derived by the oracle and
+    matched as text; the suffix count lies within 6 sigma of 2^32 / 64^3 = 16384
+    (sigma 128)."""
     key, _ = W.single()
-    specs = [{"key": key, "seed_source": M.SEED_CHACHA20, "needle": b"Zq9", "fold": False, "mode": M.MODE_SUFFIX,
+    specs = [{"key": key, "seed_source": source, "needle": b"Zq9", "fold": False, "mode": M.MODE_SUFFIX,
               "first": 0, "count": 1 << 32},
-             {"key": key, "seed_source": M.SEED_CHACHA20, "needle": b"meek", "fold": True, "mode": M.MODE_CONTAINS,
+             {"key": key, "seed_source": source, "needle": b"meek", "fold": True, "mode": M.MODE_CONTAINS,
               "first": 0, "count": 1 << 32}]
-    say("full 2^32 range on the default layout")
+    say(f"full 2^32 range, seed source {source}, on the default layout")
     hits, tries = gpu_default.batch_run(specs)
     assert tries == [1 << 32, 1 << 32]
-    say("open gpu W=24 batch=256 limbs=8")
-    other = M.Gpu(0, width=24, batch=256, limbs=M.LIMBS_R32)
+    say(f"open gpu W=24 batch=256 limbs={limbs}")
+    other = M.Gpu(0, width=24, batch=256, limbs=limbs)
     try:
-        say("full 2^32 range on the eight-word layout")
+        say(f"full 2^32 range, seed source {source}, on layout {limbs}")
         hits8, tries8 = other.batch_run(specs)
     finally:
         other.close()
@@ -151,6 +155,6 @@ def test_full_range_hit_sets_of_two_field_layouts(gpu_default, oracle):
     assert len(substring) > 100000
     assert [(h.search, h.index, h.seed, h.pk) for h in hits] == [(h.search, h.index, h.seed, h.pk) for h in hits8]
     for h in suffix[::97] + substring[::97]:
-        assert h.pk == oracle.derive(key, 0, h.index, 1)
+        assert h.pk == oracle.derive(key, source, h.index, 1)
         hay = T.hay(h.pk)
         assert hay.endswith("Zq9") if h.search == 0 else "meek" in hay.lower()
