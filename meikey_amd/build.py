@@ -87,12 +87,20 @@ def hostcheck_path(layout: str = "") -> str:
     return os.path.join(HERE, f"libmeikey_hostcheck{layout}.so")
 
 
-def build_hostcheck(out: str = HOSTCHECK) -> str:
+def build_hostcheck(out: str = HOSTCHECK, sanitize: str | None = None) -> str:
+    """sanitize: None, or a host sanitizer of the CPU build ("undefined": any undefined
+    behaviour aborts the process; "address": run Python with
+    LD_PRELOAD=$(g++ -print-file-name=libasan.so) ASAN_OPTIONS=detect_leaks=0)."""
+    extra = []
+    if sanitize:
+        extra = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=" + sanitize]
+        if sanitize == "undefined":
+            extra.append("-fno-sanitize-recover=undefined")
     for layout, defs in HOSTCHECK_LAYOUTS.items():
         path = out if layout == "" else hostcheck_path(layout)
         tmp = path + ".tmp"
-        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-function", *defs, "-o", tmp,
-              os.path.join(CSRC, "mk_hostcheck.cpp")])
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-function", *extra, *defs, "-o",
+              tmp, os.path.join(CSRC, "mk_hostcheck.cpp")])
         os.replace(tmp, path)
     return out
 
@@ -120,5 +128,10 @@ def build_all() -> None:
 
 
 if __name__ == "__main__":
-    build_all()
-    print("built", LIB, HOSTCHECK)
+    if len(sys.argv) == 3 and sys.argv[1] == "--hostcheck-sanitize":
+        # the CPU test build only, under a host sanitizer; a plain build restores it
+        build_hostcheck(sanitize=sys.argv[2])
+        print("built", HOSTCHECK, "with -fsanitize=" + sys.argv[2])
+    else:
+        build_all()
+        print("built", LIB, HOSTCHECK)
