@@ -46,10 +46,23 @@ def _run(cmd: list[str]) -> None:
 HOST_SOURCES = [os.path.join(CSRC, "mk_confirm_host.cpp")]
 
 
-def build_hip(out: str = LIB) -> str:
+def _host_sanitizer_flags(kind: str) -> list[str]:
+    # host pass only: GPU code is never built with a sanitizer
+    flags = ["-Xarch_host", "-fsanitize=" + kind, "-Xarch_host", "-fno-omit-frame-pointer"]
+    if kind == "undefined":
+        flags += ["-Xarch_host", "-fno-sanitize-recover=undefined"]
+    return flags
+
+
+def build_hip(out: str = LIB, host_sanitize: str | None = None) -> str:
     """Each source compiled to an object in parallel (every HIP source carries its own gfx950
-    code object), then linked by hipcc."""
+    code object), then linked by hipcc. host_sanitize ("address" or "undefined") builds the
+    library's host code under that sanitizer, for CPU test runs with MEIKEY_HIP_LIB pointing
+    at `out` and the compiler's runtime preloaded
+    (LD_PRELOAD=/opt/rocm/llvm/lib/clang/<version>/lib/linux/libclang_rt.asan-x86_64.so, or
+    libclang_rt.ubsan_standalone-x86_64.so)."""
     from concurrent.futures import ThreadPoolExecutor
+    hip_flags = HIP_FLAGS + (_host_sanitizer_flags(host_sanitize) if host_sanitize else [])
     tmp = out + ".tmp"
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
@@ -58,7 +71,7 @@ def build_hip(out: str = LIB) -> str:
         obj = os.path.join(bdir, os.path.basename(src) + ".host.o")
         jobs.append((["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-c", "-o", obj, src],
                      obj))
-    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    compile_flags = [f for f in hip_flags if f != "-shared"]
     for src in HIP_SOURCES:
         obj = os.path.join(bdir, os.path.basename(src) + ".o")
         jobs.append(([HIPCC, *compile_flags, "-c", "-o", obj, src], obj))
@@ -69,7 +82,7 @@ def build_hip(out: str = LIB) -> str:
         # the host objects go straight to the linker (hipcc would read them as HIP sources)
         hip_objs = [o for _, o in jobs if not o.endswith(".host.o")]
         host_objs = [o for _, o in jobs if o.endswith(".host.o")]
-        _run([HIPCC, *HIP_FLAGS, "-o", tmp, *hip_objs, *[a for o in host_objs for a in ("-Xlinker", o)]])
+        _run([HIPCC, *hip_flags, "-o", tmp, *hip_objs, *[a for o in host_objs for a in ("-Xlinker", o)]])
     finally:
         for _, obj in jobs:
             if os.path.exists(obj):
@@ -132,6 +145,10 @@ if __name__ == "__main__":
         # the CPU test build only, under a host sanitizer; a plain build restores it
         build_hostcheck(sanitize=sys.argv[2])
         print("built", HOSTCHECK, "with -fsanitize=" + sys.argv[2])
+    elif len(sys.argv) == 4 and sys.argv[1] == "--host-sanitize":
+        # a copy of the library with its host code under a sanitizer, at the path given
+        build_hip(sys.argv[3], host_sanitize=sys.argv[2])
+        print("built", sys.argv[3], "with host -fsanitize=" + sys.argv[2])
     else:
         build_all()
         print("built", LIB, HOSTCHECK)
