@@ -640,6 +640,37 @@ def test_server_graph_replays_match_eager(gpu):
     assert out[True] == out[False]
 
 
+def test_server_units_at_192_keys_per_lane(oracle):
+    """At 192 keys per lane a serving step is claimed in three 64-key units per tile
+    (unit_keys = 64), or in whole tiles where the unit does not divide the tile (128) or
+    is the tile (192). Every request's result is its first exact hit, the same in all
+    three, and the oracle's."""
+    specs = _serve_specs(12, 31)
+    say("open gpu W=16 batch=192 blocks=1")
+    g = M.Gpu(0, width=16, batch=192, blocks=1)
+    try:
+        out = {}
+        for unit in (64, 128, 192):
+            say(f"serving in {unit}-key units")
+            with M.Server(g, max_slots=6, step_keys=1 << 21, unit_keys=unit) as srv:
+                ids = {srv.submit(s): k for k, s in enumerate(specs)}
+                res = {}
+                steps = 0
+                while len(res) < len(specs):
+                    srv.step()
+                    steps += 1
+                    for r in srv.poll():
+                        res[ids[r.request]] = r
+                    assert steps < 4000
+            out[unit] = {k: (r.status, r.index, r.pk) for k, r in res.items()}
+            if unit == 64:
+                for k, s in enumerate(specs):
+                    _served_checks(oracle, s, res[k])
+        assert out[64] == out[128] == out[192]
+    finally:
+        g.close()
+
+
 def test_server_unit_keys_checked(gpu):
     """unit_keys is 64, 128, 192 or 256 (0 means the default 128); other values are refused
     before anything is allocated."""
