@@ -158,3 +158,38 @@ def test_full_range_hit_sets_of_two_field_layouts(gpu_default, oracle, source, l
         assert h.pk == oracle.derive(key, source, h.index, 1)
         hay = T.hay(h.pk)
         assert hay.endswith("Zq9") if h.search == 0 else "meek" in hay.lower()
+
+
+def test_batched_workload_on_two_field_layouts(gpu_default, oracle):
+    """The 64-search batched workload at its full size (16 each of case-insensitive and
+    case-sensitive substrings, suffixes and regex DFAs, 2^26 candidates each, about 600 k
+    hits) on the default ten-limb kernel and on the eight-word layout, whose field
+    arithmetic shares no code with it: equal complete hit sets for every search. Every
+    499th hit is re-derived by the oracle and matched against its pattern with Python's
+    own string and re semantics."""
+    import re
+    specs, pats, keys = W.batched(1 << 26)
+    say("batched workload on the default layout")
+    hits, tries = gpu_default.batch_run(specs, as_array=True)
+    assert tries == [1 << 26] * 64
+    say("open gpu W=24 batch=256 limbs=8")
+    other = M.Gpu(0, width=24, batch=256, limbs=M.LIMBS_R32)
+    try:
+        say("batched workload on the eight-word layout")
+        hits8, tries8 = other.batch_run(specs, as_array=True)
+    finally:
+        other.close()
+    assert tries8 == tries
+    assert len(hits) > 500000 and hits.tobytes() == hits8.tobytes()
+    counts = [int((hits["search"] == i).sum()) for i in range(64)]
+    assert sum(c > 0 for c in counts) >= 48, counts  # 51: the long case-sensitive patterns find none
+    for h in hits[::499]:
+        i, index, pk = int(h["search"]), int(h["index"]), bytes(h["pk"])
+        assert pk == oracle.derive(keys[i], 0, index, 1)
+        hay, p = T.hay(pk), pats[i]
+        if p.mode == M.MODE_REGEX:
+            ok = re.search(p.text, hay, re.ASCII | (0 if p.case_sensitive else re.IGNORECASE)) is not None
+        else:
+            h2, n2 = (hay, p.text) if p.case_sensitive else (hay.lower(), p.text.lower())
+            ok = h2.endswith(n2) if p.mode == M.MODE_SUFFIX else n2 in h2
+        assert ok, (i, p.text, hay)
