@@ -210,6 +210,37 @@ def test_low_security_counter_mode(gpu, oracle):
         assert gpu.derive(key, M.SEED_COUNTER, i, 1) == oracle.derive(key, M.SEED_COUNTER, i, 1), i
 
 
+@pytest.mark.parametrize("length", [32, 33, 40, 43])
+def test_long_planted_needles(gpu, oracle, length):
+    """Needles of 32 characters and more (the matcher's 64-bit shift-and states above 32;
+    43 is the whole hay, sign character included), cut from the text of a planted key:
+    substring and suffix, both case rules. Each window's full hit set is the oracle's and
+    holds the planted index."""
+    rng = random.Random(300 + length)
+    specs, planted = [], []
+    for mode, cs in [(M.MODE_CONTAINS, True), (M.MODE_CONTAINS, False), (M.MODE_SUFFIX, True),
+                     (M.MODE_SUFFIX, False)]:
+        key = rng.randbytes(32)
+        source = rng.choice([0, 1])
+        target = rng.getrandbits(40)
+        hay = T.hay(oracle.derive(key, source, target, 1))
+        if mode == M.MODE_SUFFIX:
+            text = hay[-length:]
+        else:
+            o = rng.randrange(43 - length + 1)
+            text = hay[o:o + length]
+        if not cs:
+            text = "".join(c.swapcase() if rng.random() < 0.5 else c for c in text)
+        needle = text.encode() if cs else text.lower().encode()
+        first = target - (1 << 15) - 7
+        specs.append(_spec(key, source, needle, cs, mode, first, 1 << 16))
+        planted.append(target)
+    hits, tries = gpu.batch_run(specs)
+    for i, s in enumerate(specs):
+        want = oracle.search(s["key"], s["seed_source"], s["first"], tries[i], s["needle"], not s["fold"], s["mode"])
+        assert planted[i] in want and [h.index for h in hits if h.search == i] == want, (length, i)
+
+
 def test_sign_dependent_suffix(gpu, oracle):
     """Suffixes of 2-3 chars cover tail position 42, whose value holds the x sign; the
     exact hit set must follow the real sign."""
