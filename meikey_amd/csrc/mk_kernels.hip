@@ -134,7 +134,7 @@ __device__ static u64 mk_uniform64(u64 v) {
 
 // Largest DFA that fits a wave's staging buffers (2 x MK_STAGE_CHUNKS x 1 KB; states x 65
 // x 2 bytes): 126 states with 128-byte entries, 94 with 96-byte ones.
-#define MK_LDS_DFA_STATES ((2 * MK_STAGE_CHUNKS * 64 * 16) / 130)
+#define MK_LDS_DFA_STATES ((MK_STAGE_BUFFERS * MK_STAGE_CHUNKS * 64 * 16) / 130)
 
 // Orders this wave's LDS writes before its later LDS reads by other lanes (the per-wave
 // buffers are shared by no other wave, so no workgroup barrier is needed).
@@ -343,7 +343,7 @@ __device__ __forceinline__ void mk_wave_claims(const mk_search_args& a, mk_wave_
 extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MK_SEARCH_WAVES, MK_SEARCH_WAVES)))
     MK_K(mk_search_kernel)(mk_search_args a) {
     __shared__ u64 lmask[64];
-    __shared__ uint4 stage[2 * MK_STAGE_CHUNKS * 64];
+    __shared__ uint4 stage[MK_STAGE_BUFFERS * MK_STAGE_CHUNKS * 64];
     __shared__ mk_sha_mid smid;
     mk_wave_ctx cx;
     cx.lane = threadIdx.x;

@@ -239,6 +239,19 @@ MK_DEV void mk_wait_all() {
 #endif
 }
 
+// Staging buffers per wave: 2 (the next row's gather lands in the other buffer) or 1 (it
+// lands in the same buffer once the current entry is in registers).
+#ifndef MK_STAGE_BUFFERS
+#define MK_STAGE_BUFFERS 2
+#endif
+
+// Every LDS read issued so far has completed, and no memory access moves across.
+MK_DEV void mk_wait_lds() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#endif
+}
+
 // LDS staging of the table gathers (gfx950 global_load_lds_dwordx4): the next row's
 // entry is copied straight from HBM into LDS without passing through VGPRs, so the
 // prefetch costs no registers. `stage` is this wave's 2 x 8 KB double buffer laid out
@@ -318,6 +331,25 @@ MK_DEV void mk_mul_base_frac_staged(const u32* __restrict__ table, u32 w, const 
     i32 d = mk_next_digit(g);
     mk_stage_entry(table, 0, d, cols, stage, 0);
     i32 dn = mk_next_digit(g);
+#if MK_STAGE_BUFFERS == 1
+    mk_wait_all();
+    const mk_niels first = mk_staged_niels(stage, 0, lane, d);
+    mk_wait_lds();  // row 0's entry is in registers: its buffer takes row 1's
+    mk_stage_entry(table, 1, dn, cols, stage, 0);
+    mk_ext acc = mk_row0_extended(first);
+    MK_NO_UNROLL
+    for (u32 row = 1; row + 1 < rows; row++) {
+        row_sync();
+        mk_wait_all();
+        const mk_niels cur = mk_staged_niels(stage, 0, lane, dn);
+        dn = mk_next_digit(g);
+        mk_wait_lds();
+        mk_stage_entry(table, row + 1, dn, cols, stage, 0);
+        acc = mk_add_niels(acc, cur);
+    }
+    mk_wait_all();
+    mk_add_niels_frac(acc, mk_staged_niels(stage, 0, lane, dn), yn, yd, xn, xd);
+#else
     mk_stage_entry(table, 1, dn, cols, stage, 1);
     mk_wait_all();
     mk_ext acc = mk_row0_extended(mk_staged_niels(stage, 0, lane, d));
@@ -334,6 +366,7 @@ MK_DEV void mk_mul_base_frac_staged(const u32* __restrict__ table, u32 w, const 
     }
     mk_wait_all();
     mk_add_niels_frac(acc, mk_staged_niels(stage, (rows - 1) & 1, lane, dn), yn, yd, xn, xd);
+#endif
 }
 
 MK_DEV void mk_mul_base_frac_staged(const u32* __restrict__ table, u32 w, const u32 s[8], uint4* stage, u32 lane,
