@@ -286,29 +286,54 @@ __device__ __forceinline__ bool mk_search_claim(const mk_search_args& a, mk_wave
     mk_cand* cands = a.cands;
     const u32 cand_cap = a.cand_cap;
     u32* stop = a.stop;
-    mk_lane_tile(Keff, key, dsc.source, first, sp, mul, pre_sweep, [&](u32 jj, const u32 pk[8]) {
-        const bool cand = mk_pk_match(pk, mt, true);
+    // Append the candidates of the lanes with `cand` set (index `index` each).
+    auto append = [&](bool cand, u64 index) {
         const u64 bal = __ballot(cand);
         if (bal == 0) return;
         const u32 leader = (u32)__builtin_ctzll(bal);
         const u32 below = __builtin_amdgcn_mbcnt_hi((u32)(bal >> 32), __builtin_amdgcn_mbcnt_lo((u32)bal, 0u));
         u32 base = 0;
-        if (lane == leader) {
-            base = atomicAdd(&ctr->n_cands, (u32)__builtin_popcountll(bal));
-            if (stop_on_hit) atomicExch(&stop[search], 1u);
-        }
+        if (lane == leader) base = atomicAdd(&ctr->n_cands, (u32)__builtin_popcountll(bal));
         base = __shfl(base, (int)leader);
         if (cand) {
             const u32 slot = base + below;
             if (slot < cand_cap) {
                 cands[slot].search = search;
                 cands[slot].pad = 0;
-                cands[slot].index = first + jj;
+                cands[slot].index = index;
             } else {
                 atomicOr(&ctr->overflow, 1u);
             }
         }
+    };
+    // A stop-on-hit search needs only its lowest exact hit. Per lane that is its lowest
+    // candidate whose match holds for either x sign (the back-sweep meets a lane's keys from
+    // the highest index down, so the last such candidate seen is the lowest), appended once
+    // at the end of the claim, and the candidates whose match depends on the sign, appended
+    // as found for the exact check to decide. A pattern that matches most keys then appends
+    // about one candidate per lane instead of one per key, and the list does not overflow
+    // and drop a search's lowest candidates.
+    u64 lane_best = ~0ull;
+    mk_lane_tile(Keff, key, dsc.source, first, sp, mul, pre_sweep, [&](u32 jj, const u32 pk[8]) {
+        bool cand = mk_pk_match(pk, mt, true);
+        const u64 bal = __ballot(cand);
+        if (bal == 0) return;
+        if (stop_on_hit) {
+            if (lane == (u32)__builtin_ctzll(bal)) atomicExch(&stop[search], 1u);
+            if (cand) {
+                u32 pk1[8];
+                MK_UNROLL
+                for (int i = 0; i < 8; i++) pk1[i] = pk[i];
+                pk1[7] |= 0x80000000u;  // the x sign, bit 1 of the value at the sign position
+                if (mk_pk_match(pk, mt, false) && mk_pk_match(pk1, mt, false)) {
+                    lane_best = first + jj;
+                    cand = false;
+                }
+            }
+        }
+        append(cand, first + jj);
     }, mid, invert);
+    if (stop_on_hit) append(lane_best != ~0ull, lane_best);
     return true;
 }
 

@@ -299,10 +299,13 @@ mk_status retire_step(mk_server* s, u32 b) {
         MK_HIPS(hipMemcpy(B.h_hits, B.hits, sizeof(mk_hit) * nh, hipMemcpyDeviceToHost));
     }
     (void)g;
-    // lowest-index exact hit of each slot in this step. A full candidate list (overflow)
-    // drops candidates but not their slots' stop flags, so such a slot may finish on a
-    // later step's hit: every result is still an exact, confirmed match, only not always
-    // the lowest matching index of the range searched.
+    // lowest-index exact hit of each slot in this step. The search kernel appends, for a
+    // stop-on-hit search, each lane's lowest candidate whose match holds for either x sign
+    // and the few whose match depends on it, so even a pattern that matches every key
+    // appends about one candidate per lane of the tiles claimed before its stop flag is
+    // seen. A full candidate list would still drop candidates but not their slots' stop
+    // flags: such a slot could then finish on a later hit, exact and confirmed but not the
+    // lowest of the range searched.
     std::map<u32, mk_hit> best;
     for (u32 i = 0; i < nh; i++) {
         const mk_hit& h = B.h_hits[i];

@@ -714,6 +714,74 @@ def test_server_unit_keys_checked(gpu):
             assert srv.stats()[1] >= 1 << 20
 
 
+@pytest.mark.parametrize("overlap,use_graph", [(True, True), (True, False), (False, True)])
+def test_server_churn(gpu, oracle, overlap, use_graph):
+    """Steps under churn (two in flight with overlap; replayed graphs or eager steps): 6 slots and a DFA pool that holds three of the
+    larger DFAs, so slots and DFA memory are reused while the other step still runs;
+    arrivals every step; candidate budgets that run out; unbounded requests and random
+    others cancelled while queued or running. Every request gets exactly one result: a
+    found one is the oracle's first exact hit at or after its first index, a spent one
+    has searched at least its budget, a cancelled one was cancelled."""
+    rng = random.Random(2024)
+    alpha = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
+    big = M.Pattern("[A-Z]{2}[0-9]zz$", True, M.MODE_REGEX)
+    assert big.dfa_states > 1
+    specs = []
+    for i in range(120):
+        kind = i % 5
+        key, src, first = rng.randbytes(32), rng.choice([0, 1]), rng.getrandbits(40)
+        count = 0
+        if kind == 0:
+            p = M.Pattern("".join(rng.choice(alpha) for _ in range(2)), False)
+        elif kind == 1:
+            p = M.Pattern("".join(rng.choice(alpha) for _ in range(2)), True, M.MODE_SUFFIX)
+        elif kind == 2:
+            p = big if rng.random() < 0.3 else M.Pattern(rng.choice(["[0-9][A-F]", "^[A-P]", "Q.$", "a[a-f]"]),
+                                                            True, M.MODE_REGEX)
+        elif kind == 3:  # about 2^36 tries to a hit: the budget runs out first
+            p, count = M.Pattern("".join(rng.choice(alpha) for _ in range(6)), True, M.MODE_SUFFIX), 1 << 21
+        else:  # unbounded and as rare: cancelled
+            p = M.Pattern("".join(rng.choice(alpha) for _ in range(6)), True, M.MODE_SUFFIX)
+        specs.append((p.spec(key, first, count, seed_source=src), kind))
+    with M.Server(gpu, max_slots=6, dfa_budget=3 * len(big.needle()[0]), step_keys=1 << 20, overlap=overlap,
+                  use_graph=use_graph) as srv:
+        ids, results, cancelled = {}, {}, set()
+        waiting = list(range(len(specs)))
+        steps = 0
+        while len(results) < len(specs):
+            for _ in range(rng.randint(0, 3)):
+                if waiting:
+                    k = waiting.pop(0)
+                    ids[srv.submit(specs[k][0])] = k
+            for rid, k in list(ids.items()):
+                if k in results or rid in cancelled:
+                    continue
+                if (specs[k][1] == 4 and rng.random() < 0.2) or rng.random() < 0.002:
+                    try:
+                        srv.cancel(rid)
+                        cancelled.add(rid)
+                    except M.MeikeyError:
+                        pass  # it finished in the last step; its result is waiting to be polled
+            srv.step()
+            steps += 1
+            for r in srv.poll():
+                k = ids[r.request]
+                assert k not in results
+                results[k] = r
+            assert steps < 20000
+    for rid, k in ids.items():
+        r, (spec, kind) = results[k], specs[k]
+        if r.status == 1:
+            assert rid not in cancelled
+            _served_checks(oracle, spec, r)
+        elif r.status == 0:
+            assert spec["count"] and r.tries >= spec["count"], (k, r.tries)
+        else:
+            assert r.status == -1 and rid in cancelled, (k, r.status)
+    statuses = [r.status for r in results.values()]
+    assert statuses.count(1) >= 60 and statuses.count(0) >= 20 and statuses.count(-1) >= 20, statuses
+
+
 def test_server_budget_and_cancel(gpu):
     """Admission waits for DFA-pool memory; cancelled requests end with status -1; a
     request with a candidate budget ends with status 0 when it is spent."""
