@@ -782,6 +782,30 @@ def test_server_churn(gpu, oracle, overlap, use_graph):
     assert statuses.count(1) >= 60 and statuses.count(0) >= 20 and statuses.count(-1) >= 20, statuses
 
 
+def test_batched_stop_on_hit_dense_and_sparse(gpu, oracle):
+    """Stop-on-hit searches in one launch, three whose patterns match most keys (every
+    key, most keys, one key in 64) beside three sparse ones, in both seed sources: each
+    search's lowest returned hit is its first exact hit (the oracle's), and every returned
+    hit re-derives and matches."""
+    rng = random.Random(515)
+    pats = [M.Pattern("^[A-P]", True, M.MODE_REGEX), M.Pattern("[0-9][A-F]", True, M.MODE_REGEX),
+            M.Pattern("Q.$", True, M.MODE_REGEX), M.Pattern("k9w", False), M.Pattern("Zq", True, M.MODE_SUFFIX),
+            M.Pattern("[a-f]{2}[0-9]zz", True, M.MODE_REGEX)]
+    specs = [p.spec(rng.randbytes(32), rng.getrandbits(40), 1 << 22, seed_source=i % 2, stop_on_hit=True)
+             for i, p in enumerate(pats)]
+    hits, tries = gpu.batch_run(specs)
+    for i, s in enumerate(specs):
+        mine = [h for h in hits if h.search == i]
+        assert mine, i
+        low = min(h.index for h in mine)
+        want = oracle.search(s["key"], s["seed_source"], s["first"], low - s["first"] + 1, s["needle"], not s["fold"],
+                             s["mode"])
+        assert want and want[0] == low, (i, pats[i].text)
+        for h in mine[:8]:
+            assert h.pk == oracle.derive(s["key"], s["seed_source"], h.index, 1)
+            assert pats[i].is_match(T.public_b64(h.pk)), (i, h.index)
+
+
 def test_server_budget_and_cancel(gpu):
     """Admission waits for DFA-pool memory; cancelled requests end with status -1; a
     request with a candidate budget ends with status 0 when it is spent."""
