@@ -782,6 +782,40 @@ def test_server_churn(gpu, oracle, overlap, use_graph):
     assert statuses.count(1) >= 60 and statuses.count(0) >= 20 and statuses.count(-1) >= 20, statuses
 
 
+def test_six_hundred_searches_in_one_launch(gpu, oracle):
+    """600 searches in one launch with ragged counts (0, 1, a tile less one, a tile, a
+    tile and one, a few tiles), both seed sources and every match kind: each search covers
+    its count rounded up to whole tiles, and its full hit set is the oracle's over that
+    range."""
+    rng = random.Random(600)
+    alpha = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/"
+    tile = gpu.tile_keys
+    counts = [0, 1, tile - 1, tile, tile + 1, 3 * tile + 17]
+    specs = []
+    for i in range(600):
+        kind = i % 4
+        text = "".join(rng.choice(alpha) for _ in range(2))
+        if kind == 0:
+            p = M.Pattern(text, False)
+        elif kind == 1:
+            p = M.Pattern(text, True)
+        elif kind == 2:
+            p = M.Pattern(text, rng.random() < 0.5, M.MODE_SUFFIX)
+        else:
+            p = M.Pattern(rng.choice(["[0-9][a-f]", "^[A-D]Q", "Zq?.$", "a[a-f]b"]), True, M.MODE_REGEX)
+        specs.append(p.spec(rng.randbytes(32), rng.getrandbits(44), counts[i % len(counts)], seed_source=i % 2))
+    hits, tries = gpu.batch_run(specs)
+    assert tries == [-(-s["count"] // tile) * tile for s in specs]
+    by = {}
+    for h in hits:
+        by.setdefault(h.search, []).append(h.index)
+    assert sum(len(v) for v in by.values()) > 10000
+    for i, s in enumerate(specs):
+        want = oracle.search(s["key"], s["seed_source"], s["first"], tries[i], s["needle"], not s["fold"], s["mode"],
+                             threads=16) if tries[i] else []
+        assert by.get(i, []) == want, i
+
+
 def test_batched_stop_on_hit_dense_and_sparse(gpu, oracle):
     """Stop-on-hit searches in one launch, three whose patterns match most keys (every
     key, most keys, one key in 64) beside three sparse ones, in both seed sources: each
