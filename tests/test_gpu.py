@@ -714,9 +714,11 @@ def test_server_unit_keys_checked(gpu):
             assert srv.stats()[1] >= 1 << 20
 
 
-@pytest.mark.parametrize("overlap,use_graph", [(True, True), (True, False), (False, True)])
-def test_server_churn(gpu, oracle, overlap, use_graph):
-    """Steps under churn (two in flight with overlap; replayed graphs or eager steps): 6 slots and a DFA pool that holds three of the
+@pytest.mark.parametrize("overlap,use_graph,per_step", [(True, True, 16), (True, False, 16), (False, True, 16),
+                                                        (True, True, 2)])
+def test_server_churn(gpu, oracle, overlap, use_graph, per_step):
+    """Steps under churn (two in flight with overlap; replayed graphs or eager steps; 16 or
+    2 requests per step, so most residents wait while the least-served run): 6 slots and a DFA pool that holds three of the
     larger DFAs, so slots and DFA memory are reused while the other step still runs;
     arrivals every step; candidate budgets that run out; unbounded requests and random
     others cancelled while queued or running. Every request gets exactly one result: a
@@ -744,7 +746,7 @@ def test_server_churn(gpu, oracle, overlap, use_graph):
             p = M.Pattern("".join(rng.choice(alpha) for _ in range(6)), True, M.MODE_SUFFIX)
         specs.append((p.spec(key, first, count, seed_source=src), kind))
     with M.Server(gpu, max_slots=6, dfa_budget=3 * len(big.needle()[0]), step_keys=1 << 20, overlap=overlap,
-                  use_graph=use_graph) as srv:
+                  use_graph=use_graph, active_per_step=per_step) as srv:
         ids, results, cancelled = {}, {}, set()
         waiting = list(range(len(specs)))
         steps = 0
