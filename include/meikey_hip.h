@@ -136,7 +136,9 @@ mk_status mk_prepare(mk_gpu* g, const uint8_t key[32], int seed_source, const ui
 /* Engine::launch (gpu.rs:826, 857-887): search indices [base, base + keys_per_iter * iters)
  * and return every exact hit, ascending by index (the reference returns one maybe-hit;
  * this returns the confirmed set, x sign included). With stop_on_hit, the launch may
- * stop early after the first candidate. *n_hits is the total number of exact hits;
+ * stop early after the first candidate, and it returns the range's lowest exact hit
+ * first but not necessarily every other one (a lane reports only its lowest hit whose
+ * match holds for either x sign). *n_hits is the total number of exact hits returned;
  * MK_ERR_OVERFLOW when it exceeds `cap` or the device lists. */
 mk_status mk_launch_run(mk_launch* l, uint64_t base, uint32_t iters, mk_hit_rec* hits, uint32_t cap,
                         uint32_t* n_hits);
@@ -180,7 +182,7 @@ typedef struct {
     uint32_t fold;
     uint32_t needle_len;
     const uint8_t* needle; /* text or DFA table, as in mk_prepare */
-    uint32_t stop_on_hit;
+    uint32_t stop_on_hit; /* 1: the search's lowest exact hit is all it needs (as mk_launch_run) */
     uint32_t pad;
     uint64_t first; /* first index */
     uint64_t count; /* number of indices; rounded up to whole tiles of 64 * batch keys */
